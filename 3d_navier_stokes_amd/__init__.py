@@ -1,0 +1,216 @@
+"""Python host binding of libnsg.so (the MI355X pseudo-spectral time-step library).
+
+The package directory name starts with a digit, so load it by path::
+
+    import importlib.util, pathlib
+    spec = importlib.util.spec_from_file_location(
+        "navier_stokes_amd", "3d_navier_stokes_amd/__init__.py",
+        submodule_search_locations=["3d_navier_stokes_amd"])
+
+The binding mirrors the reference's run parameters (Solver/src/utils.c:45-67) and its integrator
+entry points: ``Solver.step`` is RK4Step (Solver/src/solver.c:505), ``Solver.rhs`` is
+NonlinearRHSBatch (:620), ``Solver.measures`` is ComputeSystemMeasurables (:1142).  Spectral arrays
+are complex128 of shape (N, N, N//2+1, 3) -- the reference AoS layout; real arrays are float64 of
+shape (N, N, N, 3).  Every call goes to the HIP library; there is no CPU fallback: if libnsg.so is
+missing or fails to load, ``load_library`` raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import NamedTuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnsg.so")
+
+SCHEMES = {"rk4cn": 0, "ifrk4": 1}
+PRECISIONS = {"fp64": 0, "fp32bulk": 1}
+ICS = {"taylor_green": 0, "shapiro": 1, "k4": 2}
+SUPPORTED_N = (8, 12, 16, 24, 32, 48, 64, 96, 128, 256, 512, 1024)
+
+ERRORS = {-1: "NSG_ERR_INVALID", -2: "NSG_ERR_OOM", -3: "NSG_ERR_DEVICE", -4: "NSG_ERR_NONFINITE",
+          -5: "NSG_ERR_BLOWUP"}
+
+
+class NSGError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [("N", ctypes.c_int * 3), ("nu", ctypes.c_double), ("visc_pow", ctypes.c_int),
+                ("dt", ctypes.c_double), ("t0", ctypes.c_double), ("scheme", ctypes.c_int),
+                ("precision", ctypes.c_int), ("cfl", ctypes.c_double), ("force_eps", ctypes.c_double),
+                ("force_kf", ctypes.c_double), ("max_speed", ctypes.c_double), ("device", ctypes.c_int)]
+
+
+class _Totals(ctypes.Structure):
+    _fields_ = [("energy", ctypes.c_double), ("enstrophy", ctypes.c_double),
+                ("palinstrophy", ctypes.c_double), ("helicity", ctypes.c_double),
+                ("dissipation", ctypes.c_double)]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load libnsg.so (raises OSError if it is absent: the HIP library is the only backend)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise OSError(f"{path} not found: build it with `make -C 3d_navier_stokes_amd/csrc`")
+    lib = ctypes.CDLL(path)
+    P = ctypes.POINTER
+    vp, dp = ctypes.c_void_p, P(ctypes.c_double)
+    lib.nsg_default_params.argtypes = [P(_Params)]
+    lib.nsg_create.argtypes = [P(_Params), P(vp)]
+    lib.nsg_destroy.argtypes = [vp]
+    lib.nsg_last_error.argtypes = [vp]
+    lib.nsg_last_error.restype = ctypes.c_char_p
+    lib.nsg_n_spect.argtypes = [vp]
+    for name in ("nsg_set_state", "nsg_get_state", "nsg_set_state_real"):
+        getattr(lib, name).argtypes = [vp, vp]
+    lib.nsg_init_state.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_uint64]
+    lib.nsg_rhs.argtypes = [vp, vp, vp]
+    lib.nsg_step.argtypes = [vp, ctypes.c_long]
+    lib.nsg_measures.argtypes = [vp, P(_Totals), vp, vp, ctypes.c_int, ctypes.c_int]
+    lib.nsg_realspace.argtypes = [vp, ctypes.c_int, vp]
+    lib.nsg_get_time.argtypes = [vp, dp, dp, dp]
+    lib.nsg_set_time.argtypes = [vp, ctypes.c_double, ctypes.c_double]
+    lib.nsg_profile_passes.argtypes = [vp, ctypes.c_int, dp, dp]
+    lib.nsg_step_bytes.argtypes = [vp]
+    lib.nsg_step_bytes.restype = ctypes.c_double
+    _lib = lib
+    return lib
+
+
+class Totals(NamedTuple):
+    energy: float
+    enstrophy: float
+    palinstrophy: float
+    helicity: float
+    dissipation: float
+
+
+def _ptr(a: np.ndarray) -> ctypes.c_void_p:
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class Solver:
+    """One periodic box resident on one GPU (a `nsg_ctx`)."""
+
+    def __init__(self, N: int, nu: float = 1.0, dt: float = 1e-3, visc_pow: int = 1, scheme: str = "rk4cn",
+                 precision: str = "fp64", cfl: float = 0.0, force_eps: float = 0.0, force_kf: float = 2.5,
+                 t0: float = 0.0, max_speed: float = 0.0, device: int = 0):
+        self.lib = load_library()
+        p = _Params()
+        self.lib.nsg_default_params(ctypes.byref(p))
+        p.N[0] = p.N[1] = p.N[2] = int(N)
+        p.nu, p.visc_pow, p.dt, p.t0 = float(nu), int(visc_pow), float(dt), float(t0)
+        p.scheme, p.precision = SCHEMES[scheme], PRECISIONS[precision]
+        p.cfl, p.force_eps, p.force_kf = float(cfl), float(force_eps), float(force_kf)
+        p.max_speed, p.device = float(max_speed), int(device)
+        self.N = int(N)
+        self.params = p
+        h = ctypes.c_void_p()
+        rc = self.lib.nsg_create(ctypes.byref(p), ctypes.byref(h))
+        self._h = h
+        if rc != 0:
+            msg = self.lib.nsg_last_error(h).decode() if h.value else "nsg_create failed"
+            if h.value:
+                self.lib.nsg_destroy(h)
+                self._h = ctypes.c_void_p()
+            raise NSGError(rc, msg)
+        self.n_spect = self.lib.nsg_n_spect(h)
+
+    # -- helpers ---------------------------------------------------------------------------
+    def _check(self, rc: int):
+        if rc != 0:
+            raise NSGError(rc, self.lib.nsg_last_error(self._h).decode())
+
+    @property
+    def spectral_shape(self):
+        return (self.N, self.N, self.N // 2 + 1, 3)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self.lib.nsg_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- state -----------------------------------------------------------------------------
+    def set_state(self, u_hat: np.ndarray):
+        a = np.ascontiguousarray(u_hat, dtype=np.complex128)
+        if a.shape != self.spectral_shape:
+            raise ValueError(f"u_hat shape {a.shape} != {self.spectral_shape}")
+        self._check(self.lib.nsg_set_state(self._h, _ptr(a)))
+
+    def get_state(self) -> np.ndarray:
+        out = np.empty(self.spectral_shape, dtype=np.complex128)
+        self._check(self.lib.nsg_get_state(self._h, _ptr(out)))
+        return out
+
+    def set_state_real(self, u: np.ndarray):
+        a = np.ascontiguousarray(u, dtype=np.float64)
+        if a.shape != (self.N,) * 3 + (3,):
+            raise ValueError("u must have shape (N, N, N, 3)")
+        self._check(self.lib.nsg_set_state_real(self._h, _ptr(a)))
+
+    def init(self, kind: str, k0: float = 4.0, energy: float = 0.5, seed: int = 1):
+        self._check(self.lib.nsg_init_state(self._h, ICS[kind], float(k0), float(energy), int(seed)))
+
+    # -- numerics --------------------------------------------------------------------------
+    def rhs(self, u_hat: np.ndarray) -> np.ndarray:
+        a = np.ascontiguousarray(u_hat, dtype=np.complex128)
+        if a.shape != self.spectral_shape:
+            raise ValueError(f"u_hat shape {a.shape} != {self.spectral_shape}")
+        out = np.empty(self.spectral_shape, dtype=np.complex128)
+        self._check(self.lib.nsg_rhs(self._h, _ptr(a), _ptr(out)))
+        return out
+
+    def step(self, nsteps: int = 1):
+        self._check(self.lib.nsg_step(self._h, int(nsteps)))
+
+    def measures(self, quirk: bool = False):
+        t = _Totals()
+        ek = np.zeros(self.n_spect)
+        ens = np.zeros(self.n_spect)
+        self._check(self.lib.nsg_measures(self._h, ctypes.byref(t), _ptr(ek), _ptr(ens), self.n_spect, int(quirk)))
+        return Totals(t.energy, t.enstrophy, t.palinstrophy, t.helicity, t.dissipation), ek, ens
+
+    def realspace(self, which: int = 0) -> np.ndarray:
+        out = np.empty((self.N,) * 3 + (3,), dtype=np.float64)
+        self._check(self.lib.nsg_realspace(self._h, int(which), _ptr(out)))
+        return out
+
+    def time(self):
+        t, dt, mx = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.nsg_get_time(self._h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(mx)))
+        return t.value, dt.value, mx.value
+
+    def set_time(self, t: float, dt: float):
+        self._check(self.lib.nsg_set_time(self._h, float(t), float(dt)))
+
+    def profile(self, iters: int = 3):
+        ms = (ctypes.c_double * 6)()
+        by = (ctypes.c_double * 5)()
+        self._check(self.lib.nsg_profile_passes(self._h, int(iters), ms, by))
+        return list(ms), list(by)
+
+    def step_bytes(self) -> float:
+        return self.lib.nsg_step_bytes(self._h)

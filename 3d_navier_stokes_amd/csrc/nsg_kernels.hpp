@@ -1,0 +1,807 @@
+// nsg_kernels.hpp -- device kernels of the pseudo-spectral time step (one periodic box per call).
+//
+// Spectral data live only on the 2/3-rule box |kx|,|ky|,kz <= kmax = floor(N/3) (the reference
+// zeroes every mode with |k| > kmax, Solver/src/solver.c:1731-1737, so no other mode is ever
+// nonzero).  Compact index of a wavenumber k in [-kmax,kmax]: k >= 0 ? k : k + NK, NK = 2 kmax + 1.
+// Component-major layouts (complex elements, kz fastest, kz padded to NZP):
+//   state / K_s : [3][NK][NK][NZP]     (state always FP64; K_s in the bulk precision)
+//   A1 (pass 1) : [6][N ][NK][NZP]     after the inverse x transform of (u_hat, w_hat)
+//   A2 (pass 2) : [6][N ][N ][NZP]     after the inverse y transform
+//   A3 (pass 3) : [3][N ][N ][NZP]     r2c_z(u x w)
+//   A4 (pass 4) : [3][N ][NK][NZP]     after the forward y transform
+// Pass 5 performs the forward x transform and the epilogue of NonlinearRHSBatch
+// (Solver/src/solver.c:689-727) fused with the RK stage bookkeeping of RK4Step (:505-608).
+#pragma once
+#include "nsg_fft.hpp"
+
+namespace nsg {
+
+constexpr double kPi = 3.14159265358979323846;
+
+struct DevScalars {
+    double dt;                   // current time step
+    double t;                    // simulation time
+    unsigned long long maxbits;  // running max of |ux|+|uy|+|uz| (bit pattern of a nonneg double)
+    double max_last;             // max of the last completed step (1/N^3 normalised)
+    long long steps;             // completed steps
+    double cfl;                  // > 0: adaptive dt
+    double dx;                   // 2 pi / N
+};
+
+// ------------------------------------------------------------------------------------------
+// per-mode helpers
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ int k_of_pos(int p, int N) { return p <= N / 2 ? p : p - N; }
+__device__ __forceinline__ int pos_of_k(int k, int N) { return k >= 0 ? k : k + N; }
+__device__ __forceinline__ int cidx_of_k(int k, int NK) { return k >= 0 ? k : k + NK; }
+__device__ __forceinline__ int k_of_cidx(int c, int kmax, int NK) { return c <= kmax ? c : c - NK; }
+
+__device__ __forceinline__ double visc_rate(double nu, int p, long long k2) {
+    double kk = (double)k2, r = kk;
+    for (int q = 1; q < p; ++q) r *= kk;
+    return nu * r;
+}
+
+enum Scheme { RK4CN = 0, IFRK4 = 1 };
+
+// stage input u~_s = alpha u_hat + beta K_{s-1}
+__device__ __forceinline__ void stage_in_coef(int scheme, int stage, double D, double dt, double& a, double& b) {
+    if (stage <= 1) { a = 1.0; b = 0.0; return; }
+    if (scheme == RK4CN) {
+        a = 1.0;
+        b = (stage == 4) ? dt : 0.5 * dt;
+    } else {
+        const double e2 = exp(-D * dt * 0.5);
+        if (stage == 2) { a = e2; b = e2 * 0.5 * dt; }
+        else if (stage == 3) { a = e2; b = 0.5 * dt; }
+        else { a = exp(-D * dt); b = dt * e2; }
+    }
+}
+
+template <typename T> __device__ __forceinline__ double2 to_d2(cpx_t<T> v) { return make_double2((double)v.x, (double)v.y); }
+template <typename T> __device__ __forceinline__ cpx_t<T> from_d2(double2 v) { return cmk<cpx_t<T>>((T)v.x, (T)v.y); }
+
+// deterministic block sum (fixed tree); red must hold blockDim.x doubles
+__device__ __forceinline__ double block_sum(double v, double* red) {
+    const int tid = threadIdx.x, n = blockDim.x;
+    red[tid] = v;
+    __syncthreads();
+    int p = 1;
+    while (p < n) p <<= 1;
+    for (int s = p >> 1; s > 0; s >>= 1) {
+        if (tid < s && tid + s < n) red[tid] += red[tid + s];
+        __syncthreads();
+    }
+    double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// ------------------------------------------------------------------------------------------
+// Pass 1: stage input, curl (Solver/src/solver.c:637-650) and inverse x transform.
+// grid (NZP/W, NK); block W*TPF; thread (t, w) = (tid / W, tid % W)
+// fields: sel 0 -> (u, w) six fields; sel 1 -> u only; sel 2 -> w only (3 fields)
+// ------------------------------------------------------------------------------------------
+struct P1Args {
+    const double2* uh;
+    const void* kprev;
+    void* out;
+    const void* twN;
+    const DevScalars* sc;
+    double* fslots;
+    double nu, kf2;
+    int visc_pow, scheme, stage, force, sel;
+    int kmax, NK, NZP;
+};
+
+template <typename T, int N, int EPT, int W>
+__global__ void __launch_bounds__(N / EPT * W) pass1_kernel(P1Args a) {
+    using C = cpx_t<T>;
+    constexpr int TPF = N / EPT;
+    __shared__ C lds[N * W];
+    __shared__ double red[N / EPT * W];
+    const int tid = threadIdx.x, w = tid % W, t = tid / W;
+    const int kz = blockIdx.x * W + w;
+    const int iy = blockIdx.y;
+    const int kmax = a.kmax, NK = a.NK, NZP = a.NZP;
+    const int ky = k_of_cidx(iy, kmax, NK);
+    const bool colok = kz <= kmax;
+    const double dt = a.sc->dt;
+    const long long plane = (long long)NK * NK * NZP;
+
+    double2 u[3][EPT];
+    double fpart = 0.0;
+#pragma unroll
+    for (int m = 0; m < EPT; ++m) {
+        const int p = t + TPF * m;
+        const int kx = k_of_pos(p, N);
+        const bool ok = colok && kx >= -kmax && kx <= kmax;
+        const long long idx = ((long long)cidx_of_k(kx, NK) * NK + iy) * NZP + kz;
+        const long long k2 = (long long)kx * kx + (long long)ky * ky + (long long)kz * kz;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) u[c][m] = ok ? a.uh[c * plane + idx] : make_double2(0.0, 0.0);
+        if (a.stage > 1 && ok) {
+            double al, be;
+            stage_in_coef(a.scheme, a.stage, visc_rate(a.nu, a.visc_pow, k2), dt, al, be);
+            const cpx_t<T>* kp = (const cpx_t<T>*)a.kprev;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                double2 kv = to_d2<T>(kp[c * plane + idx]);
+                u[c][m] = make_double2(al * u[c][m].x + be * kv.x, al * u[c][m].y + be * kv.y);
+            }
+        }
+        if (a.force && ok && k2 > 0 && (double)k2 <= a.kf2) {
+            const double wgt = (kz == 0) ? 1.0 : 2.0;
+            fpart += wgt * (u[0][m].x * u[0][m].x + u[0][m].y * u[0][m].y + u[1][m].x * u[1][m].x +
+                            u[1][m].y * u[1][m].y + u[2][m].x * u[2][m].x + u[2][m].y * u[2][m].y);
+        }
+    }
+    if (a.force && blockIdx.x == 0 && (double)ky * ky <= a.kf2) {
+        double s = block_sum(fpart, red);
+        if (tid == 0) a.fslots[iy] = s;
+    }
+
+    const int f0 = (a.sel == 2) ? 3 : 0;
+    const int f1 = (a.sel == 1) ? 3 : 6;
+    C* out = (C*)a.out;
+    const C* tw = (const C*)a.twN;
+    const long long oplane = (long long)N * NK * NZP;
+    for (int f = f0; f < f1; ++f) {
+        C v[EPT];
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) {
+            double2 r;
+            if (f < 3) {
+                r = u[f][m];
+            } else {
+                const double kx = (double)k_of_pos(t + TPF * m, N), kyd = (double)ky, kzd = (double)kz;
+                // w = i k x u
+                double xr, xi;
+                if (f == 3) { xr = kyd * u[2][m].x - kzd * u[1][m].x; xi = kyd * u[2][m].y - kzd * u[1][m].y; }
+                else if (f == 4) { xr = kzd * u[0][m].x - kx * u[2][m].x; xi = kzd * u[0][m].y - kx * u[2][m].y; }
+                else { xr = kx * u[1][m].x - kyd * u[0][m].x; xi = kx * u[1][m].y - kyd * u[0][m].y; }
+                r = make_double2(-xi, xr);
+            }
+            v[m] = from_d2<T>(r);
+        }
+        fft<N, EPT, +1>(v, t, lds, w, W, tw);
+        const int fo = f - f0;
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) {
+            const int p = t + TPF * m;
+            out[fo * oplane + ((long long)p * NK + iy) * NZP + kz] = v[m];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Passes 2 and 4: column transforms along y.
+//   DIR=+1 (pass 2): rows in = compact ky (NK), rows out = all N positions
+//   DIR=-1 (pass 4): rows in = all N positions, rows out = compact ky
+// in/out layouts [F][N(x)][rows][NZP]; grid (NZP/W, N, F)
+// ------------------------------------------------------------------------------------------
+struct PYArgs {
+    const void* in;
+    void* out;
+    const void* twN;
+    int kmax, NK, NZP;
+};
+
+template <typename T, int N, int EPT, int W, int DIR>
+__global__ void __launch_bounds__(N / EPT * W) passy_kernel(PYArgs a) {
+    using C = cpx_t<T>;
+    constexpr int TPF = N / EPT;
+    __shared__ C lds[N * W];
+    const int tid = threadIdx.x, w = tid % W, t = tid / W;
+    const int kz = blockIdx.x * W + w;
+    const int x = blockIdx.y, f = blockIdx.z;
+    const int kmax = a.kmax, NK = a.NK, NZP = a.NZP;
+    const int rin = (DIR > 0) ? NK : N, rout = (DIR > 0) ? N : NK;
+    const C* in = (const C*)a.in + ((long long)f * N + x) * rin * NZP;
+    C* out = (C*)a.out + ((long long)f * N + x) * rout * NZP;
+    C v[EPT];
+#pragma unroll
+    for (int m = 0; m < EPT; ++m) {
+        const int p = t + TPF * m;
+        if constexpr (DIR > 0) {
+            const int k = k_of_pos(p, N);
+            const bool ok = k >= -kmax && k <= kmax;
+            v[m] = ok ? in[(long long)cidx_of_k(k, NK) * NZP + kz] : cmk<C>(0, 0);
+        } else {
+            v[m] = in[(long long)p * NZP + kz];
+        }
+    }
+    fft<N, EPT, DIR>(v, t, lds, w, W, (const C*)a.twN);
+#pragma unroll
+    for (int m = 0; m < EPT; ++m) {
+        const int p = t + TPF * m;
+        if constexpr (DIR > 0) {
+            out[(long long)p * NZP + kz] = v[m];
+        } else {
+            const int k = k_of_pos(p, N);
+            if (k >= -kmax && k <= kmax) out[(long long)cidx_of_k(k, NK) * NZP + kz] = v[m];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Pass 3: per (x,y) line along z.
+//  mode ZM_RHS : c2r of the six fields (u, w), u x w in registers (Solver/src/solver.c:664-677),
+//                r2c of the three products; also the running max of |ux|+|uy|+|uz|
+//  mode ZM_REAL: c2r of three fields, stored as real AoS [x][y][z][3] scaled by `scale`
+//  mode ZM_R2C : r2c of a real AoS [x][y][z][3] field
+// Half-length transforms (M = N/2): c2r forms Z[k] = (X[k] + X*[M-k]) + i e^{2pi i k/N}(X[k] - X*[M-k])
+// and z = IDFT_M(Z) gives x[2n] = Re z[n], x[2n+1] = Im z[n]; r2c packs z[n] = x[2n] + i x[2n+1],
+// Z = DFT_M(z), X[k] = (Z[k] + Z*[M-k])/2 - (i/2) e^{-2pi i k/N} (Z[k] - Z*[M-k]).
+// grid ceil(N*N/LPB); block LPB*TPFZ
+// ------------------------------------------------------------------------------------------
+enum ZMode { ZM_RHS = 0, ZM_REAL = 1, ZM_R2C = 2 };
+
+struct PZArgs {
+    const void* in;   // A2 (RHS, REAL) or real AoS doubles (R2C)
+    void* out;        // A3 (RHS, R2C) or real AoS doubles (REAL)
+    const void* twM;  // length N/2 table
+    const void* twN;  // length N table
+    DevScalars* sc;
+    double scale;
+    int kmax, NZP;
+    int track_max;
+};
+
+template <typename T, int N, int EPTZ>
+__device__ __forceinline__ void z_c2r_field(cpx_t<T> (&v)[EPTZ], const cpx_t<T>* line, int kmax, int t,
+                                            cpx_t<T>* lds, int base, const cpx_t<T>* twM, const cpx_t<T>* twN) {
+    using C = cpx_t<T>;
+    constexpr int M = N / 2, TPF = M / EPTZ;
+#pragma unroll
+    for (int m = 0; m < EPTZ; ++m) {
+        const int k = t + TPF * m;
+        const int kr = M - k;
+        C xk = (k <= kmax) ? line[k] : cmk<C>(0, 0);
+        C xr = (kr <= kmax) ? line[kr] : cmk<C>(0, 0);
+        if (k == 0) { xk.y = 0; xr.y = 0; }  // c2r uses only the real parts of X[0], X[N/2]
+        const C xrc = cconj(xr);
+        const C s = cadd(xk, xrc), d = csub(xk, xrc);
+        C e = twN[k];
+        e.y = -e.y;  // e^{+2 pi i k / N}
+        v[m] = cadd(s, cmul_i<+1>(cmul(e, d)));
+    }
+    fft<M, EPTZ, +1>(v, t, lds, base, 1, twM);
+}
+
+template <typename T, int N, int EPTZ>
+__device__ __forceinline__ void z_r2c_store(cpx_t<T> (&v)[EPTZ], cpx_t<T>* outline, int kmax, int NZP, int t,
+                                            cpx_t<T>* lds, int base, const cpx_t<T>* twM, const cpx_t<T>* twN,
+                                            bool live) {
+    using C = cpx_t<T>;
+    using R = decltype(C::x);
+    constexpr int M = N / 2, TPF = M / EPTZ;
+    fft<M, EPTZ, -1>(v, t, lds, base, 1, twM);
+#pragma unroll
+    for (int m = 0; m < EPTZ; ++m) lds[base + t + TPF * m] = v[m];
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < EPTZ; ++m) {
+        const int k = t + TPF * m;
+        if (live && k < NZP) {
+            C r = cmk<C>(0, 0);
+            if (k <= kmax) {
+                const C zk = v[m];
+                const C zr = cconj(lds[base + ((M - k) % M)]);
+                const C s = cadd(zk, zr), d = csub(zk, zr);
+                const C e = twN[k];  // e^{-2 pi i k / N}
+                const C ed = cmul(e, d);
+                // X = s/2 - (i/2) e d
+                r = cmk<C>(R(0.5) * (s.x + ed.y), R(0.5) * (s.y - ed.x));
+            }
+            outline[k] = r;
+        }
+    }
+    if (live)
+        for (int k = M + t; k < NZP; k += TPF) outline[k] = cmk<C>(0, 0);
+    __syncthreads();
+}
+
+template <typename T, int N, int EPTZ, int LPB, int MODE>
+__global__ void __launch_bounds__(N / 2 / EPTZ * LPB) passz_kernel(PZArgs a) {
+    using C = cpx_t<T>;
+    using R = decltype(C::x);
+    constexpr int M = N / 2, TPF = M / EPTZ;
+    __shared__ C lds[M * LPB];
+    const int tid = threadIdx.x, line = tid / TPF, t = tid % TPF;
+    long long lid = (long long)blockIdx.x * LPB + line;
+    const bool live = lid < (long long)N * N;
+    if (!live) lid = (long long)N * N - 1;  // idle lines recompute the last line but never store
+    const int base = line * M;
+    const int kmax = a.kmax, NZP = a.NZP;
+    const C* twM = (const C*)a.twM;
+    const C* twN = (const C*)a.twN;
+    const long long fstride = (long long)N * N * NZP;
+
+    if constexpr (MODE == ZM_RHS) {
+        const C* in = (const C*)a.in + lid * NZP;
+        C* out = (C*)a.out + lid * NZP;
+        C uu[3][EPTZ], ww[3][EPTZ];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) z_c2r_field<T, N, EPTZ>(uu[c], in + c * fstride, kmax, t, lds, base, twM, twN);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) z_c2r_field<T, N, EPTZ>(ww[c], in + (3 + c) * fstride, kmax, t, lds, base, twM, twN);
+        if (a.track_max) {
+            double mx = 0.0;
+#pragma unroll
+            for (int m = 0; m < EPTZ; ++m) {
+                double s0 = fabs((double)uu[0][m].x) + fabs((double)uu[1][m].x) + fabs((double)uu[2][m].x);
+                double s1 = fabs((double)uu[0][m].y) + fabs((double)uu[1][m].y) + fabs((double)uu[2][m].y);
+                mx = fmax(mx, fmax(s0, s1));
+            }
+            mx *= a.scale;
+            for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+            if ((tid & 63) == 0 && live) atomicMax(&a.sc->maxbits, (unsigned long long)__double_as_longlong(mx));
+        }
+        C pr[3][EPTZ];
+#pragma unroll
+        for (int m = 0; m < EPTZ; ++m) {
+            // real and imaginary parts are two independent grid points
+            pr[0][m] = cmk<C>(uu[1][m].x * ww[2][m].x - uu[2][m].x * ww[1][m].x, uu[1][m].y * ww[2][m].y - uu[2][m].y * ww[1][m].y);
+            pr[1][m] = cmk<C>(uu[2][m].x * ww[0][m].x - uu[0][m].x * ww[2][m].x, uu[2][m].y * ww[0][m].y - uu[0][m].y * ww[2][m].y);
+            pr[2][m] = cmk<C>(uu[0][m].x * ww[1][m].x - uu[1][m].x * ww[0][m].x, uu[0][m].y * ww[1][m].y - uu[1][m].y * ww[0][m].y);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 3; ++c) z_r2c_store<T, N, EPTZ>(pr[c], out + c * fstride, kmax, NZP, t, lds, base, twM, twN, live);
+    } else if constexpr (MODE == ZM_REAL) {
+        const C* in = (const C*)a.in + lid * NZP;
+        double* out = (double*)a.out + lid * (long long)N * 3;
+        for (int c = 0; c < 3; ++c) {
+            C v[EPTZ];
+            z_c2r_field<T, N, EPTZ>(v, in + c * fstride, kmax, t, lds, base, twM, twN);
+            if (live) {
+#pragma unroll
+                for (int m = 0; m < EPTZ; ++m) {
+                    const int n = t + TPF * m;
+                    out[(2 * n) * 3 + c] = (double)v[m].x * a.scale;
+                    out[(2 * n + 1) * 3 + c] = (double)v[m].y * a.scale;
+                }
+            }
+            __syncthreads();
+        }
+    } else {  // ZM_R2C
+        const double* in = (const double*)a.in + lid * (long long)N * 3;
+        C* out = (C*)a.out + lid * NZP;
+        for (int c = 0; c < 3; ++c) {
+            C v[EPTZ];
+#pragma unroll
+            for (int m = 0; m < EPTZ; ++m) {
+                const int n = t + TPF * m;
+                v[m] = cmk<C>((R)in[(2 * n) * 3 + c], (R)in[(2 * n + 1) * 3 + c]);
+            }
+            z_r2c_store<T, N, EPTZ>(v, out + c * fstride, kmax, NZP, t, lds, base, twM, twN, live);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Pass 5: forward x transform + epilogue.
+//  EP_STAGE: NL = r2c(...)/N^6, Leray projection (k=0 -> 0), 2/3 mask, forcing, then
+//            stage < 4: K_s = NL;  stage 4: u_hat = gamma u_hat + sum delta_i K_i
+//  EP_STORE: u_hat = scale * X masked (initial conditions from real space)
+// grid (NZP/W, NK); block W*TPF
+// ------------------------------------------------------------------------------------------
+enum EpMode { EP_STAGE = 0, EP_STORE = 1 };
+
+struct P5Args {
+    const void* in;   // A4
+    double2* uh;      // state
+    const void* k1;   // K1..K3 (bulk precision)
+    const void* k2;
+    const void* k3;
+    void* kout;       // K_s output (stage < 4)
+    const void* twN;
+    const DevScalars* sc;
+    const double* fslots;
+    double nu, kf2, eps_in, norm;
+    int visc_pow, scheme, stage, force;
+    int kmax, NK, NZP;
+};
+
+template <typename T, int N, int EPT, int W, int MODE>
+__global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
+    using C = cpx_t<T>;
+    constexpr int TPF = N / EPT;
+    __shared__ C lds[N * W];
+    const int tid = threadIdx.x, w = tid % W, t = tid / W;
+    const int kz = blockIdx.x * W + w;
+    const int iy = blockIdx.y;
+    const int kmax = a.kmax, NK = a.NK, NZP = a.NZP;
+    const int ky = k_of_cidx(iy, kmax, NK);
+    const long long plane = (long long)NK * NK * NZP;
+    const long long iplane = (long long)N * NK * NZP;
+    const C* in = (const C*)a.in;
+    const C* tw = (const C*)a.twN;
+
+    double2 r[3][EPT];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        C v[EPT];
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) v[m] = in[c * iplane + ((long long)(t + TPF * m) * NK + iy) * NZP + kz];
+        fft<N, EPT, -1>(v, t, lds, w, W, tw);
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) r[c][m] = to_d2<T>(v[m]);
+    }
+
+    double ffac = 0.0;
+    if (MODE == EP_STAGE && a.force) {
+        // forced-band energy of the stage input, summed over the pass-1 slots in fixed order
+        double s = 0.0;
+        int kfi = (int)floor(sqrt(a.kf2));
+        if (kfi > kmax) kfi = kmax;
+        for (int q = -kfi; q <= kfi; ++q) s += a.fslots[cidx_of_k(q, NK)];
+        const double N3 = (double)N * N * N;
+        const double Ef = 0.5 * s / (N3 * N3);
+        ffac = (Ef > 0.0) ? a.eps_in / (2.0 * Ef) : 0.0;
+    }
+    const double dt = a.sc->dt;
+    const long long kcut2 = (long long)kmax * kmax;
+
+#pragma unroll
+    for (int m = 0; m < EPT; ++m) {
+        const int p = t + TPF * m;
+        const int kx = k_of_pos(p, N);
+        if (kx < -kmax || kx > kmax || kz >= NZP) continue;
+        const long long idx = ((long long)cidx_of_k(kx, NK) * NK + iy) * NZP + kz;
+        const long long k2 = (long long)kx * kx + (long long)ky * ky + (long long)kz * kz;
+        const bool keep = k2 <= kcut2 && kz <= kmax;
+        double2 nl[3];
+        if (MODE == EP_STORE) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                nl[c] = keep ? make_double2(r[c][m].x * a.norm, r[c][m].y * a.norm) : make_double2(0.0, 0.0);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) a.uh[c * plane + idx] = nl[c];
+            continue;
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) nl[c] = make_double2(r[c][m].x * a.norm, r[c][m].y * a.norm);
+        if (k2 != 0 && keep) {
+            const double kinv = 1.0 / (double)k2;
+            const double kxd = kx, kyd = ky, kzd = kz;
+            const double dre = kxd * nl[0].x + kyd * nl[1].x + kzd * nl[2].x;
+            const double dim = kxd * nl[0].y + kyd * nl[1].y + kzd * nl[2].y;
+            nl[0].x -= kxd * kinv * dre; nl[0].y -= kxd * kinv * dim;
+            nl[1].x -= kyd * kinv * dre; nl[1].y -= kyd * kinv * dim;
+            nl[2].x -= kzd * kinv * dre; nl[2].y -= kzd * kinv * dim;
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) nl[c] = make_double2(0.0, 0.0);
+        }
+        const double D = visc_rate(a.nu, a.visc_pow, k2);
+        if (a.force && keep && k2 > 0 && (double)k2 <= a.kf2) {
+            double al, be;
+            stage_in_coef(a.scheme, a.stage, D, dt, al, be);
+            const C* kp = (const C*)(a.stage == 2 ? a.k1 : a.stage == 3 ? a.k2 : a.k3);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                double2 us = a.uh[c * plane + idx];
+                if (a.stage > 1) {
+                    double2 kv = to_d2<T>(kp[c * plane + idx]);
+                    us = make_double2(al * us.x + be * kv.x, al * us.y + be * kv.y);
+                }
+                nl[c].x += ffac * us.x;
+                nl[c].y += ffac * us.y;
+            }
+        }
+        if (a.stage < 4) {
+            C* ko = (C*)a.kout;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) ko[c * plane + idx] = from_d2<T>(nl[c]);
+        } else {
+            constexpr double B1 = 1.0 / 6.0, B2 = 1.0 / 3.0, B3 = 1.0 / 3.0, B4 = 1.0 / 6.0;
+            double g, h = 0.0, e2 = 0.0;
+            if (a.scheme == RK4CN) {
+                const double Dd = D * dt;
+                g = (2.0 - Dd) / (2.0 + Dd);
+                h = 2.0 * dt / (2.0 + Dd);
+            } else {
+                g = exp(-D * dt);
+                e2 = exp(-D * dt * 0.5);
+            }
+            const C* K1 = (const C*)a.k1;
+            const C* K2 = (const C*)a.k2;
+            const C* K3 = (const C*)a.k3;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double2 u0 = a.uh[c * plane + idx];
+                const double2 q1 = to_d2<T>(K1[c * plane + idx]);
+                const double2 q2 = to_d2<T>(K2[c * plane + idx]);
+                const double2 q3 = to_d2<T>(K3[c * plane + idx]);
+                double2 un;
+                if (a.scheme == RK4CN) {
+                    // u (2-D)/(2+D) + 2dt/(2+D) (K1/6 + K2/3 + K3/3 + K4/6), Solver/src/solver.c:601-603
+                    un.x = u0.x * g + h * (B1 * q1.x + B2 * q2.x + B3 * q3.x + B4 * nl[c].x);
+                    un.y = u0.y * g + h * (B1 * q1.y + B2 * q2.y + B3 * q3.y + B4 * nl[c].y);
+                } else {
+                    // E u + dt/6 (E K1 + 2 E2 (K2 + K3) + K4)
+                    un.x = g * u0.x + dt / 6.0 * (g * q1.x + 2.0 * e2 * (q2.x + q3.x) + nl[c].x);
+                    un.y = g * u0.y + dt / 6.0 * (g * q1.y + 2.0 * e2 * (q2.y + q3.y) + nl[c].y);
+                }
+                if (!keep) un = make_double2(0.0, 0.0);
+                a.uh[c * plane + idx] = un;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Step bookkeeping (single-thread kernels; no memsets so the step can be graph-captured)
+// ------------------------------------------------------------------------------------------
+__global__ void step_begin_kernel(DevScalars* sc) {
+    if (threadIdx.x == 0) sc->maxbits = 0ull;
+}
+__global__ void cfl_kernel(DevScalars* sc) {
+    if (threadIdx.x == 0) {
+        const double mx = __longlong_as_double((long long)sc->maxbits);
+        if (sc->cfl > 0.0 && mx > 0.0) sc->dt = sc->cfl * sc->dx / mx;
+    }
+}
+__global__ void step_end_kernel(DevScalars* sc) {
+    if (threadIdx.x == 0) {
+        sc->t += sc->dt;
+        sc->steps += 1;
+        sc->max_last = __longlong_as_double((long long)sc->maxbits);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Layout conversion: reference AoS [N][N][Nzf][3] (interleaved complex) <-> compact state
+// ------------------------------------------------------------------------------------------
+__global__ void aos_to_state_kernel(const double2* __restrict__ aos, double2* __restrict__ uh, int N, int kmax, int NK, int NZP) {
+    const long long plane = (long long)NK * NK * NZP;
+    const long long total = plane;
+    const long long kcut2 = (long long)kmax * kmax;
+    for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < total; n += (long long)gridDim.x * blockDim.x) {
+        const int kz = (int)(n % NZP);
+        const int iy = (int)((n / NZP) % NK);
+        const int ix = (int)(n / ((long long)NZP * NK));
+        const int kx = k_of_cidx(ix, kmax, NK), ky = k_of_cidx(iy, kmax, NK);
+        const long long k2 = (long long)kx * kx + (long long)ky * ky + (long long)kz * kz;
+        const bool keep = kz <= kmax && k2 <= kcut2;
+        const int Nzf = N / 2 + 1;
+        const long long src = (((long long)pos_of_k(kx, N) * N + pos_of_k(ky, N)) * Nzf + kz) * 3;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) uh[c * plane + n] = keep ? aos[src + c] : make_double2(0.0, 0.0);
+    }
+}
+
+template <typename CT>
+__global__ void compact_to_aos_kernel(const CT* __restrict__ src, double2* __restrict__ aos, int N, int kmax, int NK, int NZP) {
+    const int Nzf = N / 2 + 1;
+    const long long total = (long long)N * N * Nzf;
+    const long long plane = (long long)NK * NK * NZP;
+    for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < total; n += (long long)gridDim.x * blockDim.x) {
+        const int kz = (int)(n % Nzf);
+        const int j = (int)((n / Nzf) % N);
+        const int i = (int)(n / ((long long)Nzf * N));
+        const int kx = k_of_pos(i, N), ky = k_of_pos(j, N);
+        const bool in = kz <= kmax && kx >= -kmax && kx <= kmax && ky >= -kmax && ky <= kmax;
+        const long long s = ((long long)cidx_of_k(kx, NK) * NK + cidx_of_k(ky, NK)) * NZP + kz;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double2 v = make_double2(0.0, 0.0);
+            if (in) { CT q = src[c * plane + s]; v = make_double2((double)q.x, (double)q.y); }
+            aos[n * 3 + c] = v;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Diagnostics: ComputeSystemMeasurables (Solver/src/solver.c:1142-1276), one block per shell b
+// (modes with round(|k|) = b, i.e. k^2 in [b^2-b+1, b^2+b]); deterministic fixed-order sums.
+// part[b*7 + {0..6}] = {E, Ens, Pal, Hel, Diss, E_b, Ens_b} before the global scale factors
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ long long isqrt_ll(long long v) {
+    if (v <= 0) return 0;
+    long long s = (long long)sqrt((double)v);
+    while (s * s > v) --s;
+    while ((s + 1) * (s + 1) <= v) ++s;
+    return s;
+}
+
+__global__ void measures_kernel(const double2* __restrict__ uh, double* __restrict__ part, int N, int kmax, int NK,
+                                int NZP, double nu, int visc_pow, int quirk) {
+    __shared__ double red[256];
+    const int b = blockIdx.x;
+    const long long lo = (b == 0) ? 0 : (long long)b * b - b + 1;
+    const long long hi = (b == 0) ? 0 : (long long)b * b + b;
+    const long long plane = (long long)NK * NK * NZP;
+    const int Nzf = N / 2 + 1;
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int q = threadIdx.x; q < NK * NK; q += blockDim.x) {
+        const int ix = q / NK, iy = q % NK;
+        const long long kx = k_of_cidx(ix, kmax, NK), ky = k_of_cidx(iy, kmax, NK);
+        const long long kh2 = kx * kx + ky * ky;
+        if (kh2 > hi) continue;
+        const long long zlo_sq = lo - kh2;
+        long long zlo = (zlo_sq <= 0) ? 0 : isqrt_ll(zlo_sq - 1) + 1;  // smallest kz with kz^2 >= lo - kh2
+        long long zhi = isqrt_ll(hi - kh2);
+        if (zhi > kmax) zhi = kmax;
+        for (long long kz = zlo; kz <= zhi; ++kz) {
+            const long long idx = ((long long)ix * NK + iy) * NZP + kz;
+            const double kxd = (double)kx, kyd = (double)ky, kzd = (double)kz;
+            double2 u[3], w[3], c[3];
+            for (int d = 0; d < 3; ++d) u[d] = uh[d * plane + idx];
+            {
+                double ar = kyd * u[2].x - kzd * u[1].x, ai = kyd * u[2].y - kzd * u[1].y;
+                double br = kzd * u[0].x - kxd * u[2].x, bi = kzd * u[0].y - kxd * u[2].y;
+                double cr = kxd * u[1].x - kyd * u[0].x, ci = kxd * u[1].y - kyd * u[0].y;
+                w[0] = make_double2(-ai, ar); w[1] = make_double2(-bi, br); w[2] = make_double2(-ci, cr);
+            }
+            {
+                double ar = kyd * w[2].x - kzd * w[1].x, ai = kyd * w[2].y - kzd * w[1].y;
+                double br = kzd * w[0].x - kxd * w[2].x, bi = kzd * w[0].y - kxd * w[2].y;
+                double cr = kxd * w[1].x - kyd * w[0].x, ci = kxd * w[1].y - kyd * w[0].y;
+                c[0] = make_double2(-ai, ar); c[1] = make_double2(-bi, br); c[2] = make_double2(-ci, cr);
+            }
+            double au[3], aw[3], ac[3];
+            for (int d = 0; d < 3; ++d) {
+                au[d] = u[d].x * u[d].x + u[d].y * u[d].y;
+                aw[d] = w[d].x * w[d].x + w[d].y * w[d].y;
+                ac[d] = c[d].x * c[d].x + c[d].y * c[d].y;
+            }
+            const long long k2 = kh2 + kz * kz;
+            const double pre = visc_rate(nu, visc_pow, k2);
+            double hel = 0.0;
+            if (quirk) for (int d = 0; d < 3; ++d) hel += u[d].x * w[d].x - u[d].y * w[d].y;
+            else for (int d = 0; d < 3; ++d) hel += u[d].x * w[d].x + u[d].y * w[d].y;
+            const double U = au[0] + au[1] + au[2], Wv = aw[0] + aw[1] + aw[2], Cc = ac[0] + ac[1] + ac[2];
+            const bool edge = (kz == 0 || kz == Nzf - 1);
+            if (edge) {
+                acc[4] += pre * U; acc[1] += Wv; acc[2] += Cc; acc[3] += hel; acc[0] += U;
+            } else if (quirk) {
+                acc[4] += 2.0 * pre * U;
+                acc[1] += 2.0 * aw[0] + aw[1] + aw[2];
+                acc[2] += 2.0 * ac[0] + ac[1] + ac[2];
+                acc[3] += 2.0 * hel;
+                acc[0] += 2.0 * au[0] + au[1] + au[2];
+            } else {
+                acc[4] += 2.0 * pre * U; acc[1] += 2.0 * Wv; acc[2] += 2.0 * Cc; acc[3] += 2.0 * hel; acc[0] += 2.0 * U;
+            }
+            const double wg = edge ? 1.0 : 2.0;
+            acc[5] += wg * U;
+            acc[6] += wg * Wv;
+        }
+    }
+    for (int j = 0; j < 7; ++j) {
+        double s = block_sum(acc[j], red);
+        if (threadIdx.x == 0) part[b * 7 + j] = s;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Initial conditions
+// ------------------------------------------------------------------------------------------
+// real-space fields written as AoS [x][y][z][3]: kind 0 = Taylor-Green (Solver/src/solver.c:1553-1574),
+// kind 1 = Shapiro with the divergence-free sin(Mz) (Solver/src/solver.c:1575-1600, D17 corrected)
+__global__ void ic_real_kernel(double* __restrict__ u, int N, int kind, double nu, double t) {
+    const long long total = (long long)N * N * N;
+    for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < total; n += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(n % N), j = (int)((n / N) % N), i = (int)(n / ((long long)N * N));
+        const double x = 2.0 * kPi * i / N, y = 2.0 * kPi * j / N, z = 2.0 * kPi * k / N;
+        double ux, uy, uz;
+        if (kind == 0) {
+            ux = sin(x) * cos(y) * cos(z);
+            uy = -cos(x) * sin(y) * cos(z);
+            uz = 0.0;
+        } else {
+            const double A = 2.0, K = 2.0, L = 2.0, Mm = 2.0;
+            const double lam = sqrt(K * K + L * L + Mm * Mm);
+            const double dec = exp(-lam * lam * t * nu);
+            ux = -A / (K * K + L * L) * (lam * L * cos(K * x) * sin(L * y) * sin(Mm * z) + Mm * K * sin(K * x) * cos(L * y) * cos(Mm * z)) * dec;
+            uy = A / (K * K + L * L) * (lam * K * sin(K * x) * cos(L * y) * sin(Mm * z) - Mm * L * cos(K * x) * sin(L * y) * cos(Mm * z)) * dec;
+            uz = A * cos(K * x) * cos(L * y) * sin(Mm * z) * dec;
+        }
+        u[n * 3 + 0] = ux;
+        u[n * 3 + 1] = uy;
+        u[n * 3 + 2] = uz;
+    }
+}
+
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    for (int r = 0; r < 10; ++r) {
+        const unsigned long long p0 = (unsigned long long)0xD2511F53u * c[0];
+        const unsigned long long p1 = (unsigned long long)0xCD9E8D57u * c[2];
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+__device__ __forceinline__ double u01(uint32_t a, uint32_t b) {
+    unsigned long long v = ((unsigned long long)a << 21) ^ (unsigned long long)(b >> 11);
+    v &= (1ull << 53) - 1;
+    return ((double)v + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+// solenoidal random-phase k^4 exp(-2(k/k0)^2) mode (same construction and counters as the oracle)
+__device__ void k4_mode(int N, long long kx, long long ky, long long kz, double k0, unsigned long long seed, double2 out[3]) {
+    const int Nzf = N / 2 + 1;
+    const long long k2 = kx * kx + ky * ky + kz * kz;
+    const long long cut = N / 3;
+    out[0] = out[1] = out[2] = make_double2(0.0, 0.0);
+    if (k2 == 0 || k2 > cut * cut) return;
+    const long long i = kx >= 0 ? kx : kx + N, j = ky >= 0 ? ky : ky + N;
+    const unsigned long long n = ((unsigned long long)i * N + (unsigned long long)j) * Nzf + (unsigned long long)kz;
+    const uint32_t key0 = (uint32_t)seed, key1 = (uint32_t)(seed >> 32);
+    uint32_t r0[4] = {(uint32_t)n, (uint32_t)(n >> 32), 0u, 0u};
+    uint32_t r1[4] = {(uint32_t)n, (uint32_t)(n >> 32), 1u, 0u};
+    philox4x32_10(r0, key0, key1);
+    philox4x32_10(r1, key0, key1);
+    const double th1 = 2.0 * kPi * u01(r0[0], r0[1]);
+    const double th2 = 2.0 * kPi * u01(r0[2], r0[3]);
+    const double phi = 2.0 * kPi * u01(r1[0], r1[1]);
+    const double kk = sqrt((double)k2);
+    const double Ek = (double)k2 * (double)k2 * exp(-2.0 * (kk / k0) * (kk / k0));
+    const double A = sqrt(Ek / (4.0 * kPi * (double)k2));
+    const double ar = A * cos(th1) * cos(phi), ai = A * sin(th1) * cos(phi);
+    const double br = A * cos(th2) * sin(phi), bi = A * sin(th2) * sin(phi);
+    double e1[3], e2[3];
+    const long long kh2 = kx * kx + ky * ky;
+    if (kh2 == 0) {
+        e1[0] = 1; e1[1] = 0; e1[2] = 0; e2[0] = 0; e2[1] = 1; e2[2] = 0;
+    } else {
+        const double kh = sqrt((double)kh2);
+        e1[0] = ky / kh; e1[1] = -kx / kh; e1[2] = 0.0;
+        e2[0] = kx * kz / (kk * kh); e2[1] = ky * kz / (kk * kh); e2[2] = -(double)kh2 / (kk * kh);
+    }
+    for (int d = 0; d < 3; ++d) out[d] = make_double2(ar * e1[d] + br * e2[d], ai * e1[d] + bi * e2[d]);
+}
+
+// writes unscaled modes into the state; part[blockIdx] = block energy sum (half-spectrum weights)
+__global__ void ic_k4_kernel(double2* __restrict__ uh, double* __restrict__ part, int N, int kmax, int NK, int NZP,
+                             double k0, unsigned long long seed) {
+    __shared__ double red[256];
+    const long long plane = (long long)NK * NK * NZP;
+    const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    double e = 0.0;
+    if (n < plane) {
+        const int kz = (int)(n % NZP), iy = (int)((n / NZP) % NK), ix = (int)(n / ((long long)NZP * NK));
+        const long long kx = k_of_cidx(ix, kmax, NK), ky = k_of_cidx(iy, kmax, NK);
+        double2 v[3];
+        if (kz > kmax) {
+            v[0] = v[1] = v[2] = make_double2(0.0, 0.0);
+        } else if (kz == 0 && (ky < 0 || (ky == 0 && kx < 0))) {
+            k4_mode(N, -kx, -ky, 0, k0, seed, v);
+            for (int d = 0; d < 3; ++d) v[d].y = -v[d].y;
+        } else {
+            k4_mode(N, kx, ky, kz, k0, seed, v);
+        }
+        const double wg = (kz == 0) ? 1.0 : 2.0;
+        for (int d = 0; d < 3; ++d) {
+            uh[d * plane + n] = v[d];
+            e += wg * (v[d].x * v[d].x + v[d].y * v[d].y);
+        }
+    }
+    const double s = block_sum(e, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void sum_parts_kernel(const double* __restrict__ part, int n, double* __restrict__ out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) s += part[i];
+        out[0] = s;
+    }
+}
+
+__global__ void scale_state_kernel(double2* __restrict__ uh, long long n, const double* __restrict__ esum, double energy, double N3) {
+    const double e_now = 0.5 * esum[0];
+    const double s = (e_now > 0.0) ? sqrt(energy / e_now) * N3 : 0.0;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        uh[i].x *= s;
+        uh[i].y *= s;
+    }
+}
+
+}  // namespace nsg

@@ -1,0 +1,108 @@
+/*
+ * nsg.h -- C ABI of the MI355X (gfx950) pseudo-spectral Navier-Stokes time-step library.
+ *
+ * One context owns one periodic box [0,2pi)^3 resident in one GPU's HBM.  Plain pointers and
+ * sizes only; spectral host arrays use the reference layout: interleaved complex double
+ * (re,im) in AoS order [Nx][Ny][Nz/2+1][3], index 3*((i*Ny+j)*(Nz/2+1)+k)+d, FFTW normalisation
+ * u_hat = sum_x u(x) e^{-i k.x} (Solver/src/solver.c:640-647, :1573).  Real host arrays are AoS
+ * [Nx][Ny][Nz][3] doubles.  All calls return NSG_OK (0) or a negative NSG_ERR_* code; the text of
+ * the last error is nsg_last_error(ctx).  A context is not thread-safe; use one per host thread.
+ *
+ * The library keeps only the modes retained by the reference's 2/3 rule
+ * (|k| <= floor(N/3), Solver/src/solver.c:1731-1737): nsg_set_state applies that mask on upload,
+ * exactly as the reference applies ApplyDealiasing to every state it creates.
+ */
+#ifndef NSG_H
+#define NSG_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NSG_OK 0
+#define NSG_ERR_INVALID (-1)   /* bad argument / unsupported size */
+#define NSG_ERR_OOM (-2)       /* device or host allocation failed */
+#define NSG_ERR_DEVICE (-3)    /* HIP runtime error */
+#define NSG_ERR_NONFINITE (-4) /* NaN/Inf in the state or its diagnostics */
+#define NSG_ERR_BLOWUP (-5)    /* max|u| exceeded nsg_params.max_speed */
+
+#define NSG_SCHEME_RK4CN 0 /* reference RK4 with Crank-Nicolson viscous factor, Solver/src/solver.c:505-608 */
+#define NSG_SCHEME_IFRK4 1 /* integrating-factor RK4 */
+
+#define NSG_FP64 0     /* every FFT and product in FP64 */
+#define NSG_FP32BULK 1 /* FFTs, products and stage increments in FP32; state, RK update,
+                          exponentials and diagnostics in FP64 */
+
+#define NSG_IC_TAYLOR_GREEN 0 /* Solver/src/solver.c:1553-1574 */
+#define NSG_IC_SHAPIRO 1      /* Solver/src/solver.c:1575-1600 with u_z = A cos cos sin(Mz) (divergence-free) */
+#define NSG_IC_K4 2           /* random-phase solenoidal E(k) ~ k^4 exp(-2 (k/k0)^2), Philox-4x32-10 */
+
+typedef struct nsg_params {
+    int N[3];          /* grid; cubic, N in {8,12,16,24,32,48,64,96,128,256,512,1024} */
+    double nu;         /* viscosity (Solver/src/utils.c:65, -v) */
+    int visc_pow;      /* 1: nu|k|^2; 2: nu|k|^4 (reference -D__HYPER build, Solver/src/data_types.h:68-71) */
+    double dt;         /* time step (-h); initial step when cfl > 0 */
+    double t0;         /* start time (-s) */
+    int scheme;        /* NSG_SCHEME_* */
+    int precision;     /* NSG_FP64 | NSG_FP32BULK */
+    double cfl;        /* > 0: dt = cfl * (2pi/N) / max(|u_x|+|u_y|+|u_z|) each step (-c) */
+    double force_eps;  /* > 0: constant-power forcing, volume-averaged injection rate */
+    double force_kf;   /* forcing band 0 < |k| <= force_kf (at most 7) */
+    double max_speed;  /* > 0: nsg_step fails with NSG_ERR_BLOWUP when max|u| exceeds it */
+    int device;        /* HIP device ordinal */
+} nsg_params;
+
+typedef struct nsg_totals {
+    double energy;       /* TotalEnergy      (Solver/src/solver.c:1274) */
+    double enstrophy;    /* TotalEnstrophy   (:1271) */
+    double palinstrophy; /* TotalPalinstrophy(:1272) */
+    double helicity;     /* TotalHelicity    (:1273) */
+    double dissipation;  /* EnergyDissipation(:1270) */
+} nsg_totals;
+
+typedef struct nsg_ctx nsg_ctx;
+
+/* defaults of GetCMLArgs (Solver/src/utils.c:45-67): N=64^3, dt=1e-3, nu=1, RK4CN, FP64 */
+void nsg_default_params(nsg_params* p);
+/* replaces AllocateMemory + InitializeFFTWPlans + InitializeSpaceVariables (Solver/src/solver.c:65-77) */
+int nsg_create(const nsg_params* p, nsg_ctx** out);
+void nsg_destroy(nsg_ctx* ctx);
+const char* nsg_last_error(const nsg_ctx* ctx);
+/* n_spect = (int)sqrt(3 (N/2)^2) + 1 (Solver/src/solver.c:1384) */
+int nsg_n_spect(const nsg_ctx* ctx);
+
+/* state transfer (run_data->u_hat), reference AoS layout, 2/3 mask applied on upload */
+int nsg_set_state(nsg_ctx* ctx, const double* u_hat);
+int nsg_get_state(nsg_ctx* ctx, double* u_hat);
+/* u_hat = dealias(r2c(u)) from a real AoS field (the IC path, Solver/src/solver.c:1573,1630) */
+int nsg_set_state_real(nsg_ctx* ctx, const double* u);
+/* InitialConditions (Solver/src/solver.c:1537-1648); k0, energy (1/2<|u|^2>_vol), seed for NSG_IC_K4 */
+int nsg_init_state(nsg_ctx* ctx, int kind, double k0, double energy, uint64_t seed);
+
+/* NonlinearRHSBatch (Solver/src/solver.c:620-731), plus forcing when enabled:
+ * out = dealias(P(r2c(c2r(u_hat) x c2r(i k x u_hat)))/N^6) + f(u_hat) */
+int nsg_rhs(nsg_ctx* ctx, const double* u_hat, double* out);
+/* nsteps x RK4Step (Solver/src/solver.c:131 -> :505) on the device-resident state */
+int nsg_step(nsg_ctx* ctx, long nsteps);
+/* ComputeSystemMeasurables (Solver/src/solver.c:1142-1276); E_k/Ens_k may be NULL, else n_spect
+ * doubles. quirk = 1 reproduces the reference's operator-precedence (D3) and conjugate-free
+ * helicity (D14) arithmetic. */
+int nsg_measures(nsg_ctx* ctx, nsg_totals* out, double* E_k, double* Ens_k, int n_spect, int quirk);
+/* real-space u (which=0) or w (which=1), scaled 1/N^3 (Solver/src/hdf5_funcs.c:221-223), AoS [N][N][N][3] */
+int nsg_realspace(nsg_ctx* ctx, int which, double* out);
+/* simulation time, current dt, and max(|u_x|+|u_y|+|u_z|) of the last completed step */
+int nsg_get_time(nsg_ctx* ctx, double* t, double* dt, double* max_speed);
+int nsg_set_time(nsg_ctx* ctx, double t, double dt);
+
+/* Per-pass timing of one RK4 step: runs `iters` steps under HIP events on the library's stream and
+ * returns the mean duration (ms) of each kernel class: [0] pass1 inverse-x+curl, [1] pass2 inverse-y,
+ * [2] pass3 z (c2r, u x w, r2c), [3] pass4 forward-y, [4] pass5 forward-x+epilogue, [5] whole step.
+ * bytes[0..4] receive the algorithmic HBM bytes of one launch of each pass (stage 2 form). */
+int nsg_profile_passes(nsg_ctx* ctx, int iters, double* ms, double* bytes);
+/* algorithmic HBM bytes of one full RK4 step (sum over the 20 pass launches) */
+double nsg_step_bytes(const nsg_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

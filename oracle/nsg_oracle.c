@@ -1,0 +1,802 @@
+/*
+ * nsg_oracle.c -- CPU FP64 restatement of the reference time-step path (TEST INFRASTRUCTURE).
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load this library; it is
+ * the checker for libnsg.so and is never shipped inside the product path.  Each function cites the
+ * reference lines (paths relative to the reference repository root) that it restates.  The FFT is
+ * the oracle's own recursive mixed-radix transform (no FFTW in this image); it is pinned against
+ * numpy.fft (pocketfft) golden vectors and brute-force DFTs in tests/test_oracle_fft.py.
+ */
+#include "nsg_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+typedef struct { double re, im; } cplx;
+
+void oracle_set_threads(int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* grid helpers: Solver/src/solver.c:1764-1826 (wavenumbers), :1384 (n_spect)                 */
+/* ------------------------------------------------------------------------------------------ */
+int oracle_n_spect(int N) {
+    double h = (double)N / 2.0;
+    return (int)sqrt(h * h + h * h + h * h) + 1;
+}
+
+static inline int kxy_of(int i, int N) { return (i <= N / 2) ? i : i - N; }
+
+void oracle_wavenumbers(int N, int* kx, int* ky, int* kz) {
+    for (int i = 0; i < N; ++i) { kx[i] = kxy_of(i, N); ky[i] = kxy_of(i, N); }
+    for (int k = 0; k < N / 2 + 1; ++k) kz[k] = k;
+}
+
+/* cos/sin(2 pi e / n) with octant reduction so that multiples of n/8 are exact/symmetric */
+static void twiddle_exact(long e, long n, double* c, double* s) {
+    e %= n;
+    if (e < 0) e += n;
+    long q = (4 * e) / n;
+    long r = 4 * e - q * n; /* angle = pi/2 * (q + r/n) */
+    double cr, sr;
+    if (2 * r <= n) {
+        double a = (M_PI / 2.0) * (double)r / (double)n;
+        cr = cos(a); sr = sin(a);
+    } else {
+        double a = (M_PI / 2.0) * (double)(n - r) / (double)n;
+        cr = sin(a); sr = cos(a);
+    }
+    switch (q) {
+        case 0: *c = cr;  *s = sr;  break;
+        case 1: *c = -sr; *s = cr;  break;
+        case 2: *c = -cr; *s = -sr; break;
+        default: *c = sr; *s = -cr; break;
+    }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* 1-D mixed-radix complex FFT (recursive decimation in time)                                  */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct {
+    int n;
+    int nf;
+    int factors[32];
+    cplx* tw; /* tw[e] = exp(-2 pi i e / n) */
+} fft_plan;
+
+static void plan_init(fft_plan* p, int n) {
+    p->n = n;
+    p->nf = 0;
+    int m = n;
+    while (m % 4 == 0) { p->factors[p->nf++] = 4; m /= 4; }
+    while (m % 2 == 0) { p->factors[p->nf++] = 2; m /= 2; }
+    for (int f = 3; m > 1; f += 2)
+        while (m % f == 0) { p->factors[p->nf++] = f; m /= f; }
+    p->tw = (cplx*)malloc(sizeof(cplx) * (size_t)(n > 0 ? n : 1));
+    for (int e = 0; e < n; ++e) {
+        double c, s;
+        twiddle_exact(e, n, &c, &s);
+        p->tw[e].re = c;
+        p->tw[e].im = -s;
+    }
+}
+static void plan_free(fft_plan* p) { free(p->tw); }
+
+/* X[k1 + m*k2] = sum_q W_n^{q k1} W_r^{q k2} Y_q[k1], Y_q = DFT_m(x[q + r j]) */
+static void fft_rec(const cplx* in, cplx* out, int n, long istride, const fft_plan* p,
+                    long twstep, int level, int sign) {
+    if (n == 1) { out[0] = in[0]; return; }
+    const int r = p->factors[level];
+    const int m = n / r;
+    for (int q = 0; q < r; ++q)
+        fft_rec(in + q * istride, out + (long)q * m, m, istride * r, p, twstep * r, level + 1, sign);
+    const long N = p->n;
+    const long wr_step = N / r;
+    cplx t[64], y[64];
+    for (int k1 = 0; k1 < m; ++k1) {
+        for (int q = 0; q < r; ++q) {
+            cplx v = out[k1 + (long)m * q];
+            cplx w = p->tw[((long)q * k1 * twstep) % N];
+            if (sign > 0) w.im = -w.im;
+            t[q].re = v.re * w.re - v.im * w.im;
+            t[q].im = v.re * w.im + v.im * w.re;
+        }
+        for (int k2 = 0; k2 < r; ++k2) {
+            double sre = 0.0, sim = 0.0;
+            for (int q = 0; q < r; ++q) {
+                cplx w = p->tw[((long)q * k2 * wr_step) % N];
+                if (sign > 0) w.im = -w.im;
+                sre += t[q].re * w.re - t[q].im * w.im;
+                sim += t[q].re * w.im + t[q].im * w.re;
+            }
+            y[k2].re = sre;
+            y[k2].im = sim;
+        }
+        for (int k2 = 0; k2 < r; ++k2) out[k1 + (long)m * k2] = y[k2];
+    }
+}
+
+static void fft_line(const fft_plan* p, cplx* data, cplx* scratch, int sign) {
+    fft_rec(data, scratch, p->n, 1, p, 1, 0, sign);
+    memcpy(data, scratch, sizeof(cplx) * (size_t)p->n);
+}
+
+void oracle_fft1d(int n, double* data, int sign) {
+    fft_plan p;
+    plan_init(&p, n);
+    cplx* s = (cplx*)malloc(sizeof(cplx) * (size_t)n);
+    fft_line(&p, (cplx*)data, s, sign);
+    free(s);
+    plan_free(&p);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* 3-D real transforms (single component).  These restate what the six FFTW plans of           */
+/* InitializeFFTWPlans do (Solver/src/solver.c:2034-2073) with D1 corrected: real size N^3.    */
+/* ------------------------------------------------------------------------------------------ */
+void oracle_r2c3d(int N, const double* in, double* outd) {
+    const int Nzf = N / 2 + 1;
+    cplx* out = (cplx*)outd;
+    fft_plan p;
+    plan_init(&p, N);
+#pragma omp parallel
+    {
+        cplx* line = (cplx*)malloc(sizeof(cplx) * (size_t)N);
+        cplx* scr = (cplx*)malloc(sizeof(cplx) * (size_t)N);
+#pragma omp for schedule(static)
+        for (long ij = 0; ij < (long)N * N; ++ij) { /* z: real -> half complex */
+            const double* src = in + ij * N;
+            for (int k = 0; k < N; ++k) { line[k].re = src[k]; line[k].im = 0.0; }
+            fft_line(&p, line, scr, -1);
+            for (int k = 0; k < Nzf; ++k) out[ij * Nzf + k] = line[k];
+        }
+#pragma omp for schedule(static)
+        for (long ik = 0; ik < (long)N * Nzf; ++ik) { /* y */
+            long i = ik / Nzf, k = ik % Nzf;
+            for (int j = 0; j < N; ++j) line[j] = out[(i * N + j) * Nzf + k];
+            fft_line(&p, line, scr, -1);
+            for (int j = 0; j < N; ++j) out[(i * N + j) * Nzf + k] = line[j];
+        }
+#pragma omp for schedule(static)
+        for (long jk = 0; jk < (long)N * Nzf; ++jk) { /* x */
+            for (int i = 0; i < N; ++i) line[i] = out[(long)i * N * Nzf + jk];
+            fft_line(&p, line, scr, -1);
+            for (int i = 0; i < N; ++i) out[(long)i * N * Nzf + jk] = line[i];
+        }
+        free(line);
+        free(scr);
+    }
+    plan_free(&p);
+}
+
+void oracle_c2r3d(int N, const double* ind, double* out) {
+    const int Nzf = N / 2 + 1;
+    const long C = (long)N * N * Nzf;
+    cplx* w = (cplx*)malloc(sizeof(cplx) * (size_t)C);
+    memcpy(w, ind, sizeof(cplx) * (size_t)C);
+    fft_plan p;
+    plan_init(&p, N);
+#pragma omp parallel
+    {
+        cplx* line = (cplx*)malloc(sizeof(cplx) * (size_t)N);
+        cplx* scr = (cplx*)malloc(sizeof(cplx) * (size_t)N);
+#pragma omp for schedule(static)
+        for (long jk = 0; jk < (long)N * Nzf; ++jk) { /* x */
+            for (int i = 0; i < N; ++i) line[i] = w[(long)i * N * Nzf + jk];
+            fft_line(&p, line, scr, +1);
+            for (int i = 0; i < N; ++i) w[(long)i * N * Nzf + jk] = line[i];
+        }
+#pragma omp for schedule(static)
+        for (long ik = 0; ik < (long)N * Nzf; ++ik) { /* y */
+            long i = ik / Nzf, k = ik % Nzf;
+            for (int j = 0; j < N; ++j) line[j] = w[(i * N + j) * Nzf + k];
+            fft_line(&p, line, scr, +1);
+            for (int j = 0; j < N; ++j) w[(i * N + j) * Nzf + k] = line[j];
+        }
+#pragma omp for schedule(static)
+        for (long ij = 0; ij < (long)N * N; ++ij) { /* z: Hermitian extension, keep real part */
+            const cplx* src = w + ij * Nzf;
+            for (int k = 0; k < Nzf; ++k) line[k] = src[k];
+            for (int k = 1; k < N - N / 2; ++k) { line[N - k].re = src[k].re; line[N - k].im = -src[k].im; }
+            fft_line(&p, line, scr, +1);
+            for (int k = 0; k < N; ++k) out[ij * N + k] = line[k].re;
+        }
+        free(line);
+        free(scr);
+    }
+    plan_free(&p);
+    free(w);
+}
+
+/* component extraction between the AoS spectral layout [..][3] and single-component arrays */
+static void get_comp(long C, const cplx* aos, int d, cplx* one) {
+#pragma omp parallel for schedule(static)
+    for (long n = 0; n < C; ++n) one[n] = aos[3 * n + d];
+}
+static void put_comp(long C, cplx* aos, int d, const cplx* one) {
+#pragma omp parallel for schedule(static)
+    for (long n = 0; n < C; ++n) aos[3 * n + d] = one[n];
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* ApplyDealiasing, Solver/src/solver.c:1709-1756 (2/3 rule: zero |k| > floor(Nx/3)); D2 fixed */
+/* ------------------------------------------------------------------------------------------ */
+void oracle_dealias(int N, double* uhd) {
+    const int Nzf = N / 2 + 1;
+    const long cut = N / 3;
+    cplx* uh = (cplx*)uhd;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; ++i) {
+        const long kx = kxy_of(i, N);
+        for (int j = 0; j < N; ++j) {
+            const long ky = kxy_of(j, N);
+            for (int k = 0; k < Nzf; ++k) {
+                const long kz = k;
+                if (sqrt((double)(kx * kx + ky * ky + kz * kz)) > (double)cut) {
+                    long idx = ((long)i * N + j) * Nzf + k;
+                    for (int d = 0; d < 3; ++d) { uh[3 * idx + d].re = 0.0; uh[3 * idx + d].im = 0.0; }
+                }
+            }
+        }
+    }
+}
+
+/* w_hat = i k x u_hat, Solver/src/solver.c:637-650 */
+void oracle_curl(int N, const double* uhd, double* whd) {
+    const int Nzf = N / 2 + 1;
+    const cplx* uh = (const cplx*)uhd;
+    cplx* wh = (cplx*)whd;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; ++i) {
+        const double kx = kxy_of(i, N);
+        for (int j = 0; j < N; ++j) {
+            const double ky = kxy_of(j, N);
+            for (int k = 0; k < Nzf; ++k) {
+                const double kz = k;
+                long idx = ((long)i * N + j) * Nzf + k;
+                cplx ux = uh[3 * idx], uy = uh[3 * idx + 1], uz = uh[3 * idx + 2];
+                /* i * (a) = (-a.im, a.re) */
+                double ar = ky * uz.re - kz * uy.re, ai = ky * uz.im - kz * uy.im;
+                double br = kz * ux.re - kx * uz.re, bi = kz * ux.im - kx * uz.im;
+                double cr = kx * uy.re - ky * ux.re, ci = kx * uy.im - ky * ux.im;
+                wh[3 * idx + 0].re = -ai; wh[3 * idx + 0].im = ar;
+                wh[3 * idx + 1].re = -bi; wh[3 * idx + 1].im = br;
+                wh[3 * idx + 2].re = -ci; wh[3 * idx + 2].im = cr;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* NonlinearRHSBatch, Solver/src/solver.c:620-731:                                              */
+/*   w = c2r(i k x u_hat), u = c2r(u_hat); NL_hat = r2c(u x w) / (N^3)^2;                       */
+/*   NL_hat -= k (k.NL_hat)/|k|^2 (k=0 -> 0); dealias                                           */
+/* ------------------------------------------------------------------------------------------ */
+void oracle_rhs(int N, const double* uhd, double* nld) {
+    const int Nzf = N / 2 + 1;
+    const long C = (long)N * N * Nzf;
+    const long R = (long)N * N * N;
+    const cplx* uh = (const cplx*)uhd;
+    cplx* nl = (cplx*)nld;
+    cplx* one = (cplx*)malloc(sizeof(cplx) * (size_t)C);
+    cplx* wh = (cplx*)malloc(sizeof(cplx) * (size_t)C * 3);
+    double* u = (double*)malloc(sizeof(double) * (size_t)R * 3);
+    double* w = (double*)malloc(sizeof(double) * (size_t)R * 3);
+    oracle_curl(N, uhd, (double*)wh);
+    for (int d = 0; d < 3; ++d) {
+        get_comp(C, wh, d, one);
+        oracle_c2r3d(N, (const double*)one, w + d * R);
+        get_comp(C, uh, d, one);
+        oracle_c2r3d(N, (const double*)one, u + d * R);
+    }
+    free(wh);
+    /* u x w, Solver/src/solver.c:664-677 (stored back into u) */
+#pragma omp parallel for schedule(static)
+    for (long n = 0; n < R; ++n) {
+        double ux = u[n], uy = u[R + n], uz = u[2 * R + n];
+        double wx = w[n], wy = w[R + n], wz = w[2 * R + n];
+        u[n] = uy * wz - uz * wy;
+        u[R + n] = uz * wx - ux * wz;
+        u[2 * R + n] = ux * wy - uy * wx;
+    }
+    free(w);
+    for (int d = 0; d < 3; ++d) {
+        oracle_r2c3d(N, u + d * R, (double*)one);
+        put_comp(C, nl, d, one);
+    }
+    free(u);
+    free(one);
+    const double norm = 1.0 / pow((double)N * N * N, 2.0);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; ++i) {
+        const long kx = kxy_of(i, N);
+        for (int j = 0; j < N; ++j) {
+            const long ky = kxy_of(j, N);
+            for (int k = 0; k < Nzf; ++k) {
+                const long kz = k;
+                long idx = ((long)i * N + j) * Nzf + k;
+                cplx* v = nl + 3 * idx;
+                for (int d = 0; d < 3; ++d) { v[d].re *= norm; v[d].im *= norm; }
+                if (kx != 0 || ky != 0 || kz != 0) {
+                    double kinv = 1.0 / (double)(kx * kx + ky * ky + kz * kz);
+                    double dre = kx * v[0].re + ky * v[1].re + kz * v[2].re;
+                    double dim = kx * v[0].im + ky * v[1].im + kz * v[2].im;
+                    v[0].re -= kx * kinv * dre; v[0].im -= kx * kinv * dim;
+                    v[1].re -= ky * kinv * dre; v[1].im -= ky * kinv * dim;
+                    v[2].re -= kz * kinv * dre; v[2].im -= kz * kinv * dim;
+                } else {
+                    for (int d = 0; d < 3; ++d) { v[d].re = 0.0; v[d].im = 0.0; }
+                }
+            }
+        }
+    }
+    oracle_dealias(N, nld);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Constant-power forcing (north-star addition; the reference only stubs ApplyForcing at       */
+/* Solver/src/solver.c:730,1636).  f_hat = eps_in * u_hat / (2 E_f) on 0 < |k| <= kf, where    */
+/* E_f = 1/2 <|u|^2>_vol of the forced band (Parseval with half-spectrum weights), so the      */
+/* volume-averaged injection rate sum Re(u* . f) equals eps_in.                                 */
+/* ------------------------------------------------------------------------------------------ */
+static inline double half_weight(int k, int N) { return (k == 0 || k == N / 2) ? 1.0 : 2.0; }
+
+void oracle_forcing(int N, const double* uhd, double eps_in, double kf, double* nld) {
+    if (eps_in == 0.0) return;
+    const int Nzf = N / 2 + 1;
+    const cplx* uh = (const cplx*)uhd;
+    cplx* nl = (cplx*)nld;
+    const double kf2 = kf * kf;
+    const double N6 = pow((double)N, 6.0);
+    double Ef = 0.0;
+    for (int i = 0; i < N; ++i) {
+        const long kx = kxy_of(i, N);
+        for (int j = 0; j < N; ++j) {
+            const long ky = kxy_of(j, N);
+            for (int k = 0; k < Nzf; ++k) {
+                const long k2 = kx * kx + ky * ky + (long)k * k;
+                if (k2 == 0 || (double)k2 > kf2) continue;
+                long idx = ((long)i * N + j) * Nzf + k;
+                double s = 0.0;
+                for (int d = 0; d < 3; ++d) s += uh[3 * idx + d].re * uh[3 * idx + d].re + uh[3 * idx + d].im * uh[3 * idx + d].im;
+                Ef += half_weight(k, N) * s;
+            }
+        }
+    }
+    Ef = 0.5 * Ef / N6;
+    if (!(Ef > 0.0)) return;
+    const double fac = eps_in / (2.0 * Ef);
+    for (int i = 0; i < N; ++i) {
+        const long kx = kxy_of(i, N);
+        for (int j = 0; j < N; ++j) {
+            const long ky = kxy_of(j, N);
+            for (int k = 0; k < Nzf; ++k) {
+                const long k2 = kx * kx + ky * ky + (long)k * k;
+                if (k2 == 0 || (double)k2 > kf2) continue;
+                long idx = ((long)i * N + j) * Nzf + k;
+                for (int d = 0; d < 3; ++d) {
+                    nl[3 * idx + d].re += fac * uh[3 * idx + d].re;
+                    nl[3 * idx + d].im += fac * uh[3 * idx + d].im;
+                }
+            }
+        }
+    }
+}
+
+static void rhs_forced(int N, const double* uh, double* nl, double eps, double kf) {
+    oracle_rhs(N, uh, nl);
+    oracle_forcing(N, uh, eps, kf, nl);
+}
+
+/* nu |k|^{2p}: reference __HYPER uses pow(k^2, 2.0) (Solver/src/data_types.h:68-71) */
+static inline double visc_rate(double nu, int p, long k2) {
+    double kk = (double)k2;
+    double r = kk;
+    for (int q = 1; q < p; ++q) r *= kk;
+    return nu * r;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* RK4Step, Solver/src/solver.c:505-608 (coefficients :28-32), __NAVIER branch.  scheme 0 is   */
+/* the reference RK4 + Crank-Nicolson factor; scheme 1 is integrating-factor RK4 (Cox-Matthews */
+/* form with E = exp(-nu|k|^2p dt), E2 = exp(-nu|k|^2p dt/2)).                                  */
+/* ------------------------------------------------------------------------------------------ */
+void oracle_step(int N, double* uhd, double nu, int visc_pow, double dt, int scheme,
+                 double force_eps, double force_kf) {
+    const int Nzf = N / 2 + 1;
+    const long C = (long)N * N * Nzf;
+    const long n3 = 3 * C;
+    cplx* uh = (cplx*)uhd;
+    cplx* K1 = (cplx*)malloc(sizeof(cplx) * (size_t)n3);
+    cplx* K2 = (cplx*)malloc(sizeof(cplx) * (size_t)n3);
+    cplx* K3 = (cplx*)malloc(sizeof(cplx) * (size_t)n3);
+    cplx* K4 = (cplx*)malloc(sizeof(cplx) * (size_t)n3);
+    cplx* tmp = (cplx*)malloc(sizeof(cplx) * (size_t)n3);
+    double* E = (double*)malloc(sizeof(double) * (size_t)C);
+    double* E2 = (double*)malloc(sizeof(double) * (size_t)C);
+#pragma omp parallel for schedule(static)
+    for (long n = 0; n < C; ++n) {
+        long i = n / ((long)N * Nzf), j = (n / Nzf) % N, k = n % Nzf;
+        long kx = kxy_of((int)i, N), ky = kxy_of((int)j, N);
+        double D = visc_rate(nu, visc_pow, kx * kx + ky * ky + k * k);
+        if (scheme == ORACLE_RK4CN) { E[n] = D * dt; E2[n] = 0.0; }
+        else { E[n] = exp(-D * dt); E2[n] = exp(-D * dt * 0.5); }
+    }
+    const double A21 = 0.5, A32 = 0.5, A43 = 1.0;
+    const double B1 = 1.0 / 6.0, B2 = 1.0 / 3.0, B3 = 1.0 / 3.0, B4 = 1.0 / 6.0;
+
+    rhs_forced(N, uhd, (double*)K1, force_eps, force_kf);
+#pragma omp parallel for schedule(static)
+    for (long m = 0; m < n3; ++m) {
+        long n = m / 3;
+        if (scheme == ORACLE_RK4CN) {
+            tmp[m].re = uh[m].re + dt * A21 * K1[m].re;
+            tmp[m].im = uh[m].im + dt * A21 * K1[m].im;
+        } else {
+            tmp[m].re = E2[n] * (uh[m].re + 0.5 * dt * K1[m].re);
+            tmp[m].im = E2[n] * (uh[m].im + 0.5 * dt * K1[m].im);
+        }
+    }
+    rhs_forced(N, (const double*)tmp, (double*)K2, force_eps, force_kf);
+#pragma omp parallel for schedule(static)
+    for (long m = 0; m < n3; ++m) {
+        long n = m / 3;
+        if (scheme == ORACLE_RK4CN) {
+            tmp[m].re = uh[m].re + dt * A32 * K2[m].re;
+            tmp[m].im = uh[m].im + dt * A32 * K2[m].im;
+        } else {
+            tmp[m].re = E2[n] * uh[m].re + 0.5 * dt * K2[m].re;
+            tmp[m].im = E2[n] * uh[m].im + 0.5 * dt * K2[m].im;
+        }
+    }
+    rhs_forced(N, (const double*)tmp, (double*)K3, force_eps, force_kf);
+#pragma omp parallel for schedule(static)
+    for (long m = 0; m < n3; ++m) {
+        long n = m / 3;
+        if (scheme == ORACLE_RK4CN) {
+            tmp[m].re = uh[m].re + dt * A43 * K3[m].re;
+            tmp[m].im = uh[m].im + dt * A43 * K3[m].im;
+        } else {
+            tmp[m].re = E[n] * uh[m].re + dt * E2[n] * K3[m].re;
+            tmp[m].im = E[n] * uh[m].im + dt * E2[n] * K3[m].im;
+        }
+    }
+    rhs_forced(N, (const double*)tmp, (double*)K4, force_eps, force_kf);
+#pragma omp parallel for schedule(static)
+    for (long m = 0; m < n3; ++m) {
+        long n = m / 3;
+        if (scheme == ORACLE_RK4CN) {
+            const double D = E[n];
+            const double g = (2.0 - D) / (2.0 + D), h = 2.0 * dt / (2.0 + D);
+            uh[m].re = uh[m].re * g + h * (B1 * K1[m].re + B2 * K2[m].re + B3 * K3[m].re + B4 * K4[m].re);
+            uh[m].im = uh[m].im * g + h * (B1 * K1[m].im + B2 * K2[m].im + B3 * K3[m].im + B4 * K4[m].im);
+        } else {
+            const double e = E[n], e2 = E2[n];
+            uh[m].re = e * uh[m].re + dt / 6.0 * (e * K1[m].re + 2.0 * e2 * (K2[m].re + K3[m].re) + K4[m].re);
+            uh[m].im = e * uh[m].im + dt / 6.0 * (e * K1[m].im + 2.0 * e2 * (K2[m].im + K3[m].im) + K4[m].im);
+        }
+    }
+    free(K1); free(K2); free(K3); free(K4); free(tmp); free(E); free(E2);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* ComputeSystemMeasurables, Solver/src/solver.c:1142-1276.  totals5 = {E, Ens, Pal, Hel, eps}; */
+/* integrated over [0,2pi)^3 (scale 4 pi^3/N^6, eps scale 8 pi^3/N^6), half-spectrum weights   */
+/* 1 on kz=0,N/2 and 2 otherwise; spectra binned by round(|k|) (D16: bins >= n_spect dropped). */
+/* quirk=1 reproduces D3 (only |x|^2 doubled for E, Ens, Pal) and D14 (Hel without conjugate). */
+/* ------------------------------------------------------------------------------------------ */
+void oracle_measures(int N, const double* uhd, double nu, int visc_pow, int quirk,
+                     double* tot, double* Ek, double* Ensk) {
+    const int Nzf = N / 2 + 1;
+    const int ns = oracle_n_spect(N);
+    const cplx* uh = (const cplx*)uhd;
+    const double norm_fac = 0.5 / pow((double)N * N * N, 2.0);
+    const double const_fac = 8.0 * pow(M_PI, 3.0);
+    double sE = 0, sEns = 0, sPal = 0, sHel = 0, sDiss = 0;
+    if (Ek) memset(Ek, 0, sizeof(double) * (size_t)ns);
+    if (Ensk) memset(Ensk, 0, sizeof(double) * (size_t)ns);
+    for (int i = 0; i < N; ++i) {
+        const long kxi = kxy_of(i, N);
+        const double kx = (double)kxi;
+        for (int j = 0; j < N; ++j) {
+            const long kyi = kxy_of(j, N);
+            const double ky = (double)kyi;
+            for (int k = 0; k < Nzf; ++k) {
+                const double kz = (double)k;
+                long idx = ((long)i * N + j) * Nzf + k;
+                cplx u[3] = {uh[3 * idx], uh[3 * idx + 1], uh[3 * idx + 2]};
+                cplx w[3], c[3];
+                /* w = i k x u ; c = i k x w */
+                {
+                    double ar = ky * u[2].re - kz * u[1].re, ai = ky * u[2].im - kz * u[1].im;
+                    double br = kz * u[0].re - kx * u[2].re, bi = kz * u[0].im - kx * u[2].im;
+                    double cr = kx * u[1].re - ky * u[0].re, ci = kx * u[1].im - ky * u[0].im;
+                    w[0].re = -ai; w[0].im = ar; w[1].re = -bi; w[1].im = br; w[2].re = -ci; w[2].im = cr;
+                }
+                {
+                    double ar = ky * w[2].re - kz * w[1].re, ai = ky * w[2].im - kz * w[1].im;
+                    double br = kz * w[0].re - kx * w[2].re, bi = kz * w[0].im - kx * w[2].im;
+                    double cr = kx * w[1].re - ky * w[0].re, ci = kx * w[1].im - ky * w[0].im;
+                    c[0].re = -ai; c[0].im = ar; c[1].re = -bi; c[1].im = br; c[2].re = -ci; c[2].im = cr;
+                }
+                double au[3], aw[3], ac[3];
+                for (int d = 0; d < 3; ++d) {
+                    au[d] = u[d].re * u[d].re + u[d].im * u[d].im;
+                    aw[d] = w[d].re * w[d].re + w[d].im * w[d].im;
+                    ac[d] = c[d].re * c[d].re + c[d].im * c[d].im;
+                }
+                const long k2i = kxi * kxi + kyi * kyi + (long)k * k;
+                const double pre = visc_rate(nu, visc_pow, k2i);
+                double hel;
+                if (quirk) { /* Re(u . w) without conjugate (D14) */
+                    hel = 0.0;
+                    for (int d = 0; d < 3; ++d) hel += u[d].re * w[d].re - u[d].im * w[d].im;
+                } else {     /* Re(conj(u) . w) */
+                    hel = 0.0;
+                    for (int d = 0; d < 3; ++d) hel += u[d].re * w[d].re + u[d].im * w[d].im;
+                }
+                const double U = au[0] + au[1] + au[2], W = aw[0] + aw[1] + aw[2], Cc = ac[0] + ac[1] + ac[2];
+                if (k == 0 || k == Nzf - 1) {
+                    sDiss += pre * U; sEns += W; sPal += Cc; sHel += hel; sE += U;
+                } else if (quirk) {
+                    sDiss += 2.0 * pre * U;
+                    sEns += 2.0 * aw[0] + aw[1] + aw[2];
+                    sPal += 2.0 * ac[0] + ac[1] + ac[2];
+                    sHel += 2.0 * hel;
+                    sE += 2.0 * au[0] + au[1] + au[2];
+                } else {
+                    sDiss += 2.0 * pre * U; sEns += 2.0 * W; sPal += 2.0 * Cc; sHel += 2.0 * hel; sE += 2.0 * U;
+                }
+                const int b = (int)round(sqrt((double)k2i));
+                if (b < ns) {
+                    const double wgt = (k == 0 || k == Nzf - 1) ? 1.0 : 2.0;
+                    if (Ek) Ek[b] += wgt * const_fac * norm_fac * U;
+                    if (Ensk) Ensk[b] += wgt * const_fac * norm_fac * W;
+                }
+            }
+        }
+    }
+    tot[0] = sE * const_fac * norm_fac;
+    tot[1] = sEns * const_fac * norm_fac;
+    tot[2] = sPal * const_fac * norm_fac;
+    tot[3] = sHel * const_fac * norm_fac;
+    tot[4] = sDiss * 2.0 * const_fac * norm_fac;
+}
+
+/* real-space u (which=0) or w (which=1), scaled 1/N^3 like Solver/src/hdf5_funcs.c:221-223;   */
+/* output AoS [N][N][N][3]                                                                       */
+void oracle_realspace(int N, const double* uhd, int which, double* out) {
+    const int Nzf = N / 2 + 1;
+    const long C = (long)N * N * Nzf;
+    const long R = (long)N * N * N;
+    const cplx* src = (const cplx*)uhd;
+    cplx* wh = NULL;
+    if (which == 1) {
+        wh = (cplx*)malloc(sizeof(cplx) * (size_t)C * 3);
+        oracle_curl(N, uhd, (double*)wh);
+        src = wh;
+    }
+    cplx* one = (cplx*)malloc(sizeof(cplx) * (size_t)C);
+    double* r = (double*)malloc(sizeof(double) * (size_t)R);
+    const double s = 1.0 / ((double)N * N * N);
+    for (int d = 0; d < 3; ++d) {
+        get_comp(C, src, d, one);
+        oracle_c2r3d(N, (const double*)one, r);
+        for (long n = 0; n < R; ++n) out[3 * n + d] = r[n] * s;
+    }
+    free(one);
+    free(r);
+    free(wh);
+}
+
+/* max over grid points of |u_x|+|u_y|+|u_z| (real space, 1/N^3 normalised) */
+double oracle_max_speed_sum(int N, const double* uhd) {
+    const long R = (long)N * N * N;
+    double* u = (double*)malloc(sizeof(double) * (size_t)R * 3);
+    oracle_realspace(N, uhd, 0, u);
+    double m = 0.0;
+    for (long n = 0; n < R; ++n) {
+        double v = fabs(u[3 * n]) + fabs(u[3 * n + 1]) + fabs(u[3 * n + 2]);
+        if (v > m) m = v;
+    }
+    free(u);
+    return m;
+}
+
+/* CFL time step (the reference's GetTimestep is declared but undefined, Solver/src/solver.c:1319-1321):
+ * dt = cfl * dx / max(|u_x|+|u_y|+|u_z|), dx = 2 pi / N */
+double oracle_cfl_dt(int N, const double* uhd, double cfl) {
+    double m = oracle_max_speed_sum(N, uhd);
+    return cfl * (2.0 * M_PI / N) / m;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Initial conditions, Solver/src/solver.c:1537-1648                                            */
+/* ------------------------------------------------------------------------------------------ */
+static void ic_from_real(int N, const double* u /* AoS [N^3][3] */, double* uhd) {
+    const int Nzf = N / 2 + 1;
+    const long C = (long)N * N * Nzf;
+    const long R = (long)N * N * N;
+    double* r = (double*)malloc(sizeof(double) * (size_t)R);
+    cplx* one = (cplx*)malloc(sizeof(cplx) * (size_t)C);
+    for (int d = 0; d < 3; ++d) {
+        for (long n = 0; n < R; ++n) r[n] = u[3 * n + d];
+        oracle_r2c3d(N, r, (double*)one);
+        put_comp(C, (cplx*)uhd, d, one);
+    }
+    free(r);
+    free(one);
+    oracle_dealias(N, uhd);
+}
+
+/* TAYLOR_GREEN, Solver/src/solver.c:1553-1574 (KAPPA = 1) */
+void oracle_ic_taylor_green(int N, double* uhd) {
+    const long R = (long)N * N * N;
+    double* u = (double*)malloc(sizeof(double) * (size_t)R * 3);
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j)
+            for (int k = 0; k < N; ++k) {
+                double x = 2.0 * M_PI * i / N, y = 2.0 * M_PI * j / N, z = 2.0 * M_PI * k / N;
+                long n = ((long)i * N + j) * N + k;
+                u[3 * n + 0] = sin(x) * cos(y) * cos(z);
+                u[3 * n + 1] = -cos(x) * sin(y) * cos(z);
+                u[3 * n + 2] = 0.0;
+            }
+    ic_from_real(N, u, uhd);
+    free(u);
+}
+
+/* SHAPIRO, Solver/src/solver.c:1575-1600 with A=2, K=L=M=2 (Solver/src/data_types.h:139-142);  */
+/* D17 corrected: u_z uses sin(Mz), making the field divergence-free and Beltrami (w = lambda u) */
+void oracle_ic_shapiro(int N, double nu, double t, double* uhd) {
+    const double Aa = 2.0, Kk = 2.0, Ll = 2.0, Mm = 2.0;
+    const double lam = sqrt(Kk * Kk + Ll * Ll + Mm * Mm);
+    const double decay = exp(-lam * lam * t * nu);
+    const long R = (long)N * N * N;
+    double* u = (double*)malloc(sizeof(double) * (size_t)R * 3);
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j)
+            for (int k = 0; k < N; ++k) {
+                double x = 2.0 * M_PI * i / N, y = 2.0 * M_PI * j / N, z = 2.0 * M_PI * k / N;
+                long n = ((long)i * N + j) * N + k;
+                u[3 * n + 0] = -Aa / (Kk * Kk + Ll * Ll) *
+                               (lam * Ll * cos(Kk * x) * sin(Ll * y) * sin(Mm * z) + Mm * Kk * sin(Kk * x) * cos(Ll * y) * cos(Mm * z)) * decay;
+                u[3 * n + 1] = Aa / (Kk * Kk + Ll * Ll) *
+                               (lam * Kk * sin(Kk * x) * cos(Ll * y) * sin(Mm * z) - Mm * Ll * cos(Kk * x) * sin(Ll * y) * cos(Mm * z)) * decay;
+                u[3 * n + 2] = Aa * cos(Kk * x) * cos(Ll * y) * sin(Mm * z) * decay;
+            }
+    ic_from_real(N, u, uhd);
+    free(u);
+}
+
+/* default RANDOM branch, Solver/src/solver.c:1551,1605-1624: srand(123456789); per mode and     */
+/* component amplitude U[0,1] then phase U[0,1] (draw order within the expression is             */
+/* unspecified C; amplitude-first is assumed: parity unpinned), then dealias.                    */
+void oracle_ic_random_ref(int N, double* uhd) {
+    const int Nzf = N / 2 + 1;
+    cplx* uh = (cplx*)uhd;
+    srand(123456789);
+    for (long n = 0; n < (long)N * N * Nzf * 3; ++n) {
+        double amp = (double)rand() / (double)RAND_MAX;
+        double ph = (double)rand() / (double)RAND_MAX;
+        uh[n].re = amp * cos(ph * 2.0 * M_PI);
+        uh[n].im = amp * sin(ph * 2.0 * M_PI);
+    }
+    oracle_dealias(N, uhd);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), counter-based RNG shared with the GPU    */
+/* ------------------------------------------------------------------------------------------ */
+void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        c0 = hi1 ^ c1 ^ k0;
+        c1 = lo1;
+        c2 = hi0 ^ c3 ^ k1;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+static inline double u01_from(uint32_t a, uint32_t b) {
+    uint64_t v = ((uint64_t)a << 21) ^ (uint64_t)(b >> 11);
+    v &= ((uint64_t)1 << 53) - 1;
+    return ((double)v + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+/* Solenoidal random-phase field with shell spectrum E(k) ~ k^4 exp(-2 (k/k0)^2), Hermitian in
+ * the kz=0 plane, zero outside the 2/3 sphere, scaled so that 1/2 <|u|^2>_vol = energy.
+ * Per stored mode (kx,ky,kz) at linear index n=(i*N+j)*Nzf+k: Philox(ctr={n,0,0,0},key=seed) gives
+ * theta1, theta2 (from words 0-1, 2-3); Philox(ctr={n,1,0,0}) gives phi.
+ * u = alpha e1 + beta e2, alpha = A e^{i theta1} cos phi, beta = A e^{i theta2} sin phi,
+ * A = sqrt(E(k)/(4 pi k^2)), e1 = (ky,-kx,0)/kh, e2 = (kx kz, ky kz, -kh^2)/(|k| kh)
+ * (kh = 0: e1 = x^, e2 = y^).  Modes with kz=0 and (ky<0 or (ky==0 and kx<0)) take the
+ * conjugate of their mirror (-kx,-ky,0). */
+static void k4_mode(int N, long kx, long ky, long kz, double k0, uint64_t seed, cplx out[3]) {
+    const int Nzf = N / 2 + 1;
+    const long k2 = kx * kx + ky * ky + kz * kz;
+    const double cut = (double)(N / 3);
+    out[0].re = out[0].im = out[1].re = out[1].im = out[2].re = out[2].im = 0.0;
+    if (k2 == 0 || sqrt((double)k2) > cut) return;
+    const long i = (kx >= 0) ? kx : kx + N, j = (ky >= 0) ? ky : ky + N;
+    const uint64_t n = ((uint64_t)i * N + (uint64_t)j) * Nzf + (uint64_t)kz;
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t c[4] = {(uint32_t)n, (uint32_t)(n >> 32), 0u, 0u}, r0[4], r1[4];
+    oracle_philox4x32_10(c, key, r0);
+    c[2] = 1u;
+    oracle_philox4x32_10(c, key, r1);
+    const double th1 = 2.0 * M_PI * u01_from(r0[0], r0[1]);
+    const double th2 = 2.0 * M_PI * u01_from(r0[2], r0[3]);
+    const double phi = 2.0 * M_PI * u01_from(r1[0], r1[1]);
+    const double kk = sqrt((double)k2);
+    const double Ek = (double)k2 * (double)k2 * exp(-2.0 * (kk / k0) * (kk / k0));
+    const double A = sqrt(Ek / (4.0 * M_PI * (double)k2));
+    const double ar = A * cos(th1) * cos(phi), ai = A * sin(th1) * cos(phi);
+    const double br = A * cos(th2) * sin(phi), bi = A * sin(th2) * sin(phi);
+    double e1[3], e2[3];
+    const long kh2 = kx * kx + ky * ky;
+    if (kh2 == 0) {
+        e1[0] = 1; e1[1] = 0; e1[2] = 0;
+        e2[0] = 0; e2[1] = 1; e2[2] = 0;
+    } else {
+        const double kh = sqrt((double)kh2);
+        e1[0] = ky / kh; e1[1] = -kx / kh; e1[2] = 0.0;
+        e2[0] = kx * kz / (kk * kh); e2[1] = ky * kz / (kk * kh); e2[2] = -(double)kh2 / (kk * kh);
+    }
+    for (int d = 0; d < 3; ++d) {
+        out[d].re = ar * e1[d] + br * e2[d];
+        out[d].im = ai * e1[d] + bi * e2[d];
+    }
+}
+
+void oracle_ic_k4(int N, double k0, double energy, uint64_t seed, double* uhd) {
+    const int Nzf = N / 2 + 1;
+    cplx* uh = (cplx*)uhd;
+    double s = 0.0;
+    for (int i = 0; i < N; ++i) {
+        const long kx = kxy_of(i, N);
+        for (int j = 0; j < N; ++j) {
+            const long ky = kxy_of(j, N);
+            for (int k = 0; k < Nzf; ++k) {
+                cplx v[3];
+                if (k == 0 && (ky < 0 || (ky == 0 && kx < 0))) {
+                    k4_mode(N, -kx, -ky, 0, k0, seed, v);
+                    for (int d = 0; d < 3; ++d) v[d].im = -v[d].im;
+                } else {
+                    k4_mode(N, kx, ky, k, k0, seed, v);
+                }
+                long idx = ((long)i * N + j) * Nzf + k;
+                for (int d = 0; d < 3; ++d) {
+                    uh[3 * idx + d] = v[d];
+                    s += half_weight(k, N) * (v[d].re * v[d].re + v[d].im * v[d].im);
+                }
+            }
+        }
+    }
+    /* here u_hat holds continuous-amplitude coefficients; FFTW normalisation multiplies by N^3,
+     * so 1/2<|u|^2> = 1/2 * sum_full |c|^2 for c = u_hat / N^3 */
+    const double e_now = 0.5 * s;
+    const double scale = (e_now > 0.0) ? sqrt(energy / e_now) * pow((double)N, 3.0) : 0.0;
+    for (long n = 0; n < (long)N * N * Nzf * 3; ++n) { uh[n].re *= scale; uh[n].im *= scale; }
+}
