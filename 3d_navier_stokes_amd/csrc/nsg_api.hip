@@ -27,25 +27,27 @@ using namespace nsg;
 //   W64/W32 : kz columns per block in the x/y passes (128-byte row segments)
 // ------------------------------------------------------------------------------------------
 template <int N> struct Cfg;
-#define NSG_CFG(n, e64, e32, ez64, ez32)                                   \
-    template <> struct Cfg<n> {                                            \
-        static constexpr int EPT64 = e64, EPT32 = e32;                     \
-        static constexpr int EPTZ64 = ez64, EPTZ32 = ez32;                 \
-        static constexpr int W64 = 8, W32 = 16;                            \
+// EX/EY/EZ: elements per thread of the x, y (length N) and z (length N/2) transforms;
+// WX/WY: kz columns per block of the x and y passes (FP64 / FP32)
+#define NSG_CFG(n, ex64, ex32, ey64, ey32, ez64, ez32)                       \
+    template <> struct Cfg<n> {                                              \
+        static constexpr int EX64 = ex64, EX32 = ex32, EY64 = ey64, EY32 = ey32; \
+        static constexpr int EZ64 = ez64, EZ32 = ez32;                       \
+        static constexpr int WX64 = 8, WX32 = 16, WY64 = 8, WY32 = 16;       \
     };
-//      N     EPT64 EPT32 EPTZ64 EPTZ32
-NSG_CFG(8,    8,    8,    4,     4)
-NSG_CFG(12,   12,   12,   6,     6)
-NSG_CFG(16,   8,    8,    8,     8)
-NSG_CFG(24,   12,   12,   12,    12)
-NSG_CFG(32,   8,    8,    8,     8)
-NSG_CFG(48,   12,   12,   12,    12)
-NSG_CFG(64,   8,    8,    8,     8)
-NSG_CFG(96,   12,   12,   12,    12)
-NSG_CFG(128,  8,    8,    8,     8)
-NSG_CFG(256,  8,    8,    4,     8)
-NSG_CFG(512,  8,    16,   4,     8)
-NSG_CFG(1024, 16,   16,   4,     8)
+//      N     EX64 EX32 EY64 EY32 EZ64 EZ32
+NSG_CFG(8,    8,   8,   8,   8,   4,   4)
+NSG_CFG(12,   12,  12,  12,  12,  6,   6)
+NSG_CFG(16,   8,   8,   8,   8,   4,   8)
+NSG_CFG(24,   12,  12,  12,  12,  12,  12)
+NSG_CFG(32,   8,   8,   8,   8,   8,   8)
+NSG_CFG(48,   12,  12,  12,  12,  12,  12)
+NSG_CFG(64,   8,   8,   8,   8,   8,   8)
+NSG_CFG(96,   12,  12,  12,  12,  12,  12)
+NSG_CFG(128,  8,   8,   8,   8,   8,   8)
+NSG_CFG(256,  8,   8,   8,   8,   4,   8)
+NSG_CFG(512,  8,   8,   8,   16,  4,   8)
+NSG_CFG(1024, 16,  16,  16,  16,  8,   8)
 #undef NSG_CFG
 
 template <typename F> static int dispatch_N(int N, F&& f) {
@@ -69,6 +71,8 @@ template <typename F> static int dispatch_N(int N, F&& f) {
 struct nsg_ctx {
     nsg_params p;
     int N = 0, kmax = 0, NK = 0, NZP = 0, nspect = 0;
+    int ncu = 256, zblocks_per_cu = 8;  // persistent z-pass grid = ncu * zblocks_per_cu blocks
+    int zocc = 1;                       // >= 3: z kernel compiled for 3 waves/SIMD (fewer VGPRs)
     bool fp32 = false;
     hipStream_t s = nullptr;
     double2* uh = nullptr;   // state [3][NK][NK][NZP]
@@ -179,21 +183,32 @@ template <typename T> static const void* twN_of(nsg_ctx* c) {
 template <typename T> static const void* twM_of(nsg_ctx* c) {
     if constexpr (std::is_same<T, double>::value) return c->twM64; else return c->twM32;
 }
-template <typename T, int N> static constexpr int Wof() { return std::is_same<T, double>::value ? Cfg<N>::W64 : Cfg<N>::W32; }
-template <typename T, int N> static constexpr int EPTof() { return std::is_same<T, double>::value ? Cfg<N>::EPT64 : Cfg<N>::EPT32; }
-template <typename T, int N> static constexpr int EZof() { return std::is_same<T, double>::value ? Cfg<N>::EPTZ64 : Cfg<N>::EPTZ32; }
+template <typename T> static constexpr bool is64() { return std::is_same<T, double>::value; }
+template <typename T, int N> static constexpr int WXof() { return is64<T>() ? Cfg<N>::WX64 : Cfg<N>::WX32; }
+template <typename T, int N> static constexpr int WYof() { return is64<T>() ? Cfg<N>::WY64 : Cfg<N>::WY32; }
+template <typename T, int N> static constexpr int EXof() { return is64<T>() ? Cfg<N>::EX64 : Cfg<N>::EX32; }
+template <typename T, int N> static constexpr int EYof() { return is64<T>() ? Cfg<N>::EY64 : Cfg<N>::EY32; }
+template <typename T, int N> static constexpr int EZof() { return is64<T>() ? Cfg<N>::EZ64 : Cfg<N>::EZ32; }
+
+// grid of the x passes: 2-D (tiles, rows), or 1-D XCD-paired when a tile row is under 128 bytes
+template <typename T, int W> static dim3 xgrid(int ntiles, int nrows) {
+    if (W * sizeof(cpx_t<T>) >= 128) return dim3(ntiles, nrows);
+    const long long pairs = (long long)(ntiles / 2) * nrows;
+    return dim3((unsigned)(16 * ((pairs + 7) / 8)));
+}
 
 template <typename T, int N>
 static int launch_pass1(nsg_ctx* c, const double2* uh, const void* kprev, int stage, int sel, int force) {
-    constexpr int EPT = EPTof<T, N>(), W = Wof<T, N>(), TPF = N / EPT;
+    constexpr int EPT = EXof<T, N>(), W = WXof<T, N>(), TPF = N / EPT;
     static_assert(TPF * W <= 1024, "block too large");
     P1Args a;
     a.uh = uh; a.kprev = kprev; a.out = c->B1; a.twN = twN_of<T>(c); a.sc = c->sc; a.fslots = c->fslots;
     a.nu = c->p.nu; a.kf2 = c->p.force_kf * c->p.force_kf;
     a.visc_pow = c->p.visc_pow; a.scheme = c->p.scheme; a.stage = stage; a.force = force; a.sel = sel;
     a.kmax = c->kmax; a.NK = c->NK; a.NZP = c->NZP;
+    const dim3 grid = xgrid<T, W>(c->NZP / W, c->NK);
     prof_begin(c);
-    hipLaunchKernelGGL((pass1_kernel<T, N, EPT, W>), dim3(c->NZP / W, c->NK), dim3(TPF * W), 0, c->s, a);
+    hipLaunchKernelGGL((pass1_kernel<T, N, EPT, W>), grid, dim3(TPF * W), 0, c->s, a);
     LAUNCHCHK(c);
     prof_end(c, 0);
     return NSG_OK;
@@ -201,7 +216,7 @@ static int launch_pass1(nsg_ctx* c, const double2* uh, const void* kprev, int st
 
 template <typename T, int N, int DIR>
 static int launch_passy(nsg_ctx* c, const void* in, void* out, int F) {
-    constexpr int EPT = EPTof<T, N>(), W = Wof<T, N>(), TPF = N / EPT;
+    constexpr int EPT = EYof<T, N>(), W = WYof<T, N>(), TPF = N / EPT;
     static_assert(TPF * W <= 1024, "block too large");
     PYArgs a;
     a.in = in; a.out = out; a.twN = twN_of<T>(c); a.kmax = c->kmax; a.NK = c->NK; a.NZP = c->NZP;
@@ -209,6 +224,33 @@ static int launch_passy(nsg_ctx* c, const void* in, void* out, int F) {
     hipLaunchKernelGGL((passy_kernel<T, N, EPT, W, DIR>), dim3(c->NZP / W, N, F), dim3(TPF * W), 0, c->s, a);
     LAUNCHCHK(c);
     prof_end(c, DIR > 0 ? 1 : 3);
+    return NSG_OK;
+}
+
+// RHS z pass: persistent wavefronts (WPB per block) when a line fits in a wavefront
+template <typename T, int N> static constexpr bool zrhs_persistent() {
+    constexpr int TPFZ = N / 2 / EZof<T, N>();
+    return TPFZ <= 64 && 64 % TPFZ == 0 && ((N / 2) & (N / 2 - 1)) == 0;
+}
+
+template <typename T, int N>
+static int launch_passz_rhs(nsg_ctx* c, const void* in, void* out, double scale, int track_max) {
+    constexpr int EPTZ = EZof<T, N>(), TPFZ = N / 2 / EPTZ, LPW = 64 / TPFZ, WPB = 4;
+    static_assert(EPTZ >= 1 && LPW >= 1, "bad z configuration");
+    PZArgs a;
+    a.in = in; a.out = out; a.twM = twM_of<T>(c); a.twN = twN_of<T>(c); a.sc = c->sc;
+    a.scale = scale; a.kmax = c->kmax; a.NZP = c->NZP; a.track_max = track_max;
+    const long long groups = ((long long)N * N + LPW - 1) / LPW;
+    long long blocks = (groups + WPB - 1) / WPB;
+    const long long cap = (long long)c->ncu * c->zblocks_per_cu;
+    if (blocks > cap) blocks = cap;
+    prof_begin(c);
+    if (c->zocc >= 3)
+        hipLaunchKernelGGL((passz_rhs_kernel<T, N, EPTZ, WPB, 3>), dim3((unsigned)blocks), dim3(64 * WPB), 0, c->s, a);
+    else
+        hipLaunchKernelGGL((passz_rhs_kernel<T, N, EPTZ, WPB, 1>), dim3((unsigned)blocks), dim3(64 * WPB), 0, c->s, a);
+    LAUNCHCHK(c);
+    prof_end(c, 2);
     return NSG_OK;
 }
 
@@ -230,7 +272,7 @@ static int launch_passz(nsg_ctx* c, const void* in, void* out, double scale, int
 
 template <typename T, int N, int MODE>
 static int launch_pass5(nsg_ctx* c, double2* uh, int stage, void* kout, int force, double norm) {
-    constexpr int EPT = EPTof<T, N>(), W = Wof<T, N>(), TPF = N / EPT;
+    constexpr int EPT = EXof<T, N>(), W = WXof<T, N>(), TPF = N / EPT;
     static_assert(TPF * W <= 1024, "block too large");
     P5Args a;
     a.in = c->B2; a.uh = uh; a.k1 = c->K[0]; a.k2 = c->K[1]; a.k3 = c->K[2]; a.kout = kout;
@@ -238,8 +280,9 @@ static int launch_pass5(nsg_ctx* c, double2* uh, int stage, void* kout, int forc
     a.nu = c->p.nu; a.kf2 = c->p.force_kf * c->p.force_kf; a.eps_in = c->p.force_eps; a.norm = norm;
     a.visc_pow = c->p.visc_pow; a.scheme = c->p.scheme; a.stage = stage; a.force = force;
     a.kmax = c->kmax; a.NK = c->NK; a.NZP = c->NZP;
+    const dim3 grid = xgrid<T, W>(c->NZP / W, c->NK);
     prof_begin(c);
-    hipLaunchKernelGGL((pass5_kernel<T, N, EPT, W, MODE>), dim3(c->NZP / W, c->NK), dim3(TPF * W), 0, c->s, a);
+    hipLaunchKernelGGL((pass5_kernel<T, N, EPT, W, MODE>), grid, dim3(TPF * W), 0, c->s, a);
     LAUNCHCHK(c);
     prof_end(c, 4);
     return NSG_OK;
@@ -254,7 +297,11 @@ static int run_stage(nsg_ctx* c, double2* uh, int stage, bool track_max, bool fi
     int rc;
     if ((rc = launch_pass1<T, N>(c, uh, kprev, stage, 0, force))) return rc;
     if ((rc = launch_passy<T, N, +1>(c, c->B1, c->B2, 6))) return rc;
-    if ((rc = launch_passz<T, N, ZM_RHS>(c, c->B2, c->B1, 1.0 / N3, track_max ? 1 : 0))) return rc;
+    if constexpr (zrhs_persistent<T, N>()) {
+        if ((rc = launch_passz_rhs<T, N>(c, c->B2, c->B1, 1.0 / N3, track_max ? 1 : 0))) return rc;
+    } else {
+        if ((rc = launch_passz<T, N, ZM_RHS>(c, c->B2, c->B1, 1.0 / N3, track_max ? 1 : 0))) return rc;
+    }
     if (track_max && c->p.cfl > 0.0) {
         hipLaunchKernelGGL(cfl_kernel, dim3(1), dim3(1), 0, c->s, c->sc);
         LAUNCHCHK(c);
@@ -360,6 +407,15 @@ int nsg_create(const nsg_params* p, nsg_ctx** out) {
     if (e != hipSuccess) { *out = c; return fail(c, NSG_ERR_DEVICE, "hipSetDevice(%d): %s", p->device, hipGetErrorString(e)); }
     *out = c;
     HIPCHK(c, hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+    {
+        hipDeviceProp_t prop;
+        HIPCHK(c, hipGetDeviceProperties(&prop, p->device));
+        c->ncu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        const char* zb = std::getenv("NSG_ZBLOCKS_PER_CU");
+        if (zb && std::atoi(zb) > 0) c->zblocks_per_cu = std::atoi(zb);
+        const char* zo = std::getenv("NSG_ZOCC");
+        if (zo && std::atoi(zo) > 0) c->zocc = std::atoi(zo);
+    }
     HIPCHK(c, hipEventCreate(&c->ev[0]));
     HIPCHK(c, hipEventCreate(&c->ev[1]));
 

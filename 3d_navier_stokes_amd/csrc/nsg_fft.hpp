@@ -143,8 +143,26 @@ __device__ __forceinline__ void stage_compute(C (&v)[EPT], int t, const C* __res
     }
 }
 
-// write stage outputs to LDS at the Stockham output positions; lds address = base + pos*PS
-template <int L, int EPT, int TPF, int S, typename C>
+// LDS slot of position pos in the exchange that follows a stage (P, R).  With PAD, when the
+// stage writes runs of P < 16 contiguous positions spaced P*R apart, P pad slots follow every
+// P*R positions so that neighbouring lanes fall in different LDS banks.
+template <int P, int R, bool PAD> __device__ __forceinline__ int lds_slot(int pos) {
+    if constexpr (PAD && P < 16) return pos + P * (pos / (P * R));
+    else return pos;
+}
+// LDS elements one transform needs (padded exchanges need up to L + L/R slots)
+constexpr int fft_lds_len(int L, int ept, bool pad) {
+    int n = L;
+    if (!pad) return n;
+    for (int s = 0; s + 1 < num_stages(L, ept); ++s) {
+        int P = stage_p(L, ept, s), R = stage_radix(L, ept, s);
+        if (P < 16) { int m = L + P * ((L - 1) / (P * R)) + 1; if (m > n) n = m; }
+    }
+    return n;
+}
+
+// write stage outputs to LDS at the Stockham output positions; lds address = base + slot*PS
+template <int L, int EPT, int TPF, int S, bool PAD, typename C>
 __device__ __forceinline__ void stage_scatter(const C (&v)[EPT], int t, C* lds, int base, int PS) {
     constexpr int R = stage_radix(L, EPT, S);
     constexpr int P = stage_p(L, EPT, S);
@@ -155,38 +173,55 @@ __device__ __forceinline__ void stage_scatter(const C (&v)[EPT], int t, C* lds, 
         const int k = b % P;
         const int o = (b - k) * R + k;
 #pragma unroll
-        for (int q = 0; q < R; ++q) lds[base + (o + q * P) * PS] = v[beta + B * q];
+        for (int q = 0; q < R; ++q) lds[base + lds_slot<P, R, PAD>(o + q * P) * PS] = v[beta + B * q];
     }
 }
 
-template <int L, int EPT, int TPF, typename C>
+template <int L, int EPT, int TPF, int S, bool PAD, typename C>
 __device__ __forceinline__ void gather_pattern(C (&v)[EPT], int t, const C* lds, int base, int PS) {
+    constexpr int R = stage_radix(L, EPT, S);
+    constexpr int P = stage_p(L, EPT, S);
 #pragma unroll
-    for (int m = 0; m < EPT; ++m) v[m] = lds[base + (t + TPF * m) * PS];
+    for (int m = 0; m < EPT; ++m) v[m] = lds[base + lds_slot<P, R, PAD>(t + TPF * m) * PS];
 }
 
-template <int L, int EPT, int TPF, int S, int NS, int DIR, typename C>
+// LDS exchange barrier.  SYNC_BLOCK: the transform spans several wavefronts (s_barrier);
+// SYNC_WAVE: all TPF threads of a transform sit in one wavefront, which executes its LDS
+// instructions in order, so only a wavefront-scope fence is needed and waves run independently.
+enum SyncKind { SYNC_BLOCK = 0, SYNC_WAVE = 1 };
+template <int SYNC> __device__ __forceinline__ void lds_sync() {
+    if constexpr (SYNC == SYNC_BLOCK) {
+        __syncthreads();
+    } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+template <int L, int EPT, int TPF, int S, int NS, int DIR, int SYNC, bool PAD, typename C>
 __device__ __forceinline__ void fft_stages(C (&v)[EPT], int t, C* lds, int base, int PS, const C* __restrict__ tw) {
     if constexpr (S < NS) {
         stage_compute<L, EPT, TPF, S, DIR>(v, t, tw);
         if constexpr (S + 1 < NS) {
-            stage_scatter<L, EPT, TPF, S>(v, t, lds, base, PS);
-            __syncthreads();
-            gather_pattern<L, EPT, TPF>(v, t, lds, base, PS);
-            __syncthreads();
+            stage_scatter<L, EPT, TPF, S, PAD>(v, t, lds, base, PS);
+            lds_sync<SYNC>();
+            gather_pattern<L, EPT, TPF, S, PAD>(v, t, lds, base, PS);
+            lds_sync<SYNC>();
         }
-        fft_stages<L, EPT, TPF, S + 1, NS, DIR>(v, t, lds, base, PS, tw);
+        fft_stages<L, EPT, TPF, S + 1, NS, DIR, SYNC, PAD>(v, t, lds, base, PS, tw);
     }
 }
 
 // Full length-L transform of the values held in pattern t + TPF*m; result in the same pattern.
-// Every thread of the block must call it (it contains barriers) even if its column is idle.
-template <int L, int EPT, int DIR, typename C>
+// With SYNC_BLOCK every thread of the block must call it (it contains barriers).
+template <int L, int EPT, int DIR, int SYNC = SYNC_BLOCK, bool PAD = false, typename C>
 __device__ __forceinline__ void fft(C (&v)[EPT], int t, C* lds, int base, int PS, const C* __restrict__ tw) {
     static_assert(schedule_ok(L, EPT), "unsupported FFT length / elements-per-thread");
     constexpr int TPF = L / EPT;
     constexpr int NS = num_stages(L, EPT);
-    fft_stages<L, EPT, TPF, 0, NS, DIR>(v, t, lds, base, PS, tw);
+    static_assert(SYNC == SYNC_BLOCK || (TPF <= 64 && 64 % TPF == 0), "wave-synchronous transform must fit a wavefront");
+    fft_stages<L, EPT, TPF, 0, NS, DIR, SYNC, PAD>(v, t, lds, base, PS, tw);
 }
 
 }  // namespace nsg

@@ -29,6 +29,22 @@ struct DevScalars {
 };
 
 // ------------------------------------------------------------------------------------------
+// layouts
+//   state / K_s : [3][NK (ky)][NK (kx)][NZP]  -- y-major, so an x pass at fixed ky stays inside
+//                 one NK*NZP plane
+//   A1 / A4     : [F][N/XB][NK][XB][NZP]      -- x rows tiled in blocks of XB: an x pass touches
+//                 N/XB page runs instead of N, a y pass strides XB*NZP elements
+// ------------------------------------------------------------------------------------------
+template <int N> constexpr int xtile() { return (N % 16 == 0) ? 16 : 4; }
+__device__ __forceinline__ long long sidx(int ix, int iy, int kz, int NK, int NZP) {
+    return ((long long)iy * NK + ix) * NZP + kz;
+}
+template <int N> __device__ __forceinline__ long long xidx(int x, int iy, int kz, int NK, int NZP) {
+    constexpr int XB = xtile<N>();
+    return (((long long)(x / XB) * NK + iy) * XB + (x % XB)) * NZP + kz;
+}
+
+// ------------------------------------------------------------------------------------------
 // per-mode helpers
 // ------------------------------------------------------------------------------------------
 __device__ __forceinline__ int k_of_pos(int p, int N) { return p <= N / 2 ? p : p - N; }
@@ -78,6 +94,30 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 }
 
 // ------------------------------------------------------------------------------------------
+// Block -> (kz tile, row) mapping for the x passes.  When a tile row is narrower than a 128-byte
+// line (PAIR), the two tiles 2j, 2j+1 that share each line are given linear block ids L and L+8:
+// blocks are dispatched round-robin over the 8 XCDs, so both halves of every line are fetched
+// through the same XCD's L2 one dispatch round apart.  1-D grid of 16*ceil(npairs*nrows/8) blocks.
+// ------------------------------------------------------------------------------------------
+template <bool PAIR>
+__device__ __forceinline__ bool tile_of_block(int ntiles, int nrows, int& tile, int& row) {
+    if constexpr (!PAIR) {
+        tile = blockIdx.x;
+        row = blockIdx.y;
+        return true;
+    } else {
+        const int L = blockIdx.x;
+        const int grp = L >> 4, r = L & 15;
+        const int q = grp * 8 + (r & 7);
+        const int npairs = ntiles >> 1;
+        if (q >= npairs * nrows) return false;
+        tile = 2 * (q % npairs) + (r >> 3);
+        row = q / npairs;
+        return true;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
 // Pass 1: stage input, curl (Solver/src/solver.c:637-650) and inverse x transform.
 // grid (NZP/W, NK); block W*TPF; thread (t, w) = (tid / W, tid % W)
 // fields: sel 0 -> (u, w) six fields; sel 1 -> u only; sel 2 -> w only (3 fields)
@@ -98,28 +138,31 @@ template <typename T, int N, int EPT, int W>
 __global__ void __launch_bounds__(N / EPT * W) pass1_kernel(P1Args a) {
     using C = cpx_t<T>;
     constexpr int TPF = N / EPT;
+    constexpr bool PAIR = W * sizeof(C) < 128;
     __shared__ C lds[N * W];
     __shared__ double red[N / EPT * W];
+    int tile, iy;
+    if (!tile_of_block<PAIR>(a.NZP / W, a.NK, tile, iy)) return;
     const int tid = threadIdx.x, w = tid % W, t = tid / W;
-    const int kz = blockIdx.x * W + w;
-    const int iy = blockIdx.y;
+    const int kz = tile * W + w;
     const int kmax = a.kmax, NK = a.NK, NZP = a.NZP;
     const int ky = k_of_cidx(iy, kmax, NK);
     const bool colok = kz <= kmax;
     const double dt = a.sc->dt;
     const long long plane = (long long)NK * NK * NZP;
 
-    double2 u[3][EPT];
+    C u[3][EPT];  // stage input in the bulk precision (formed in FP64)
     double fpart = 0.0;
 #pragma unroll
     for (int m = 0; m < EPT; ++m) {
         const int p = t + TPF * m;
         const int kx = k_of_pos(p, N);
         const bool ok = colok && kx >= -kmax && kx <= kmax;
-        const long long idx = ((long long)cidx_of_k(kx, NK) * NK + iy) * NZP + kz;
+        const long long idx = sidx(cidx_of_k(kx, NK), iy, kz, NK, NZP);
         const long long k2 = (long long)kx * kx + (long long)ky * ky + (long long)kz * kz;
+        double2 ud[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) u[c][m] = ok ? a.uh[c * plane + idx] : make_double2(0.0, 0.0);
+        for (int c = 0; c < 3; ++c) ud[c] = ok ? a.uh[c * plane + idx] : make_double2(0.0, 0.0);
         if (a.stage > 1 && ok) {
             double al, be;
             stage_in_coef(a.scheme, a.stage, visc_rate(a.nu, a.visc_pow, k2), dt, al, be);
@@ -127,16 +170,18 @@ __global__ void __launch_bounds__(N / EPT * W) pass1_kernel(P1Args a) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 double2 kv = to_d2<T>(kp[c * plane + idx]);
-                u[c][m] = make_double2(al * u[c][m].x + be * kv.x, al * u[c][m].y + be * kv.y);
+                ud[c] = make_double2(al * ud[c].x + be * kv.x, al * ud[c].y + be * kv.y);
             }
         }
         if (a.force && ok && k2 > 0 && (double)k2 <= a.kf2) {
             const double wgt = (kz == 0) ? 1.0 : 2.0;
-            fpart += wgt * (u[0][m].x * u[0][m].x + u[0][m].y * u[0][m].y + u[1][m].x * u[1][m].x +
-                            u[1][m].y * u[1][m].y + u[2][m].x * u[2][m].x + u[2][m].y * u[2][m].y);
+            fpart += wgt * (ud[0].x * ud[0].x + ud[0].y * ud[0].y + ud[1].x * ud[1].x + ud[1].y * ud[1].y +
+                            ud[2].x * ud[2].x + ud[2].y * ud[2].y);
         }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) u[c][m] = from_d2<T>(ud[c]);
     }
-    if (a.force && blockIdx.x == 0 && (double)ky * ky <= a.kf2) {
+    if (a.force && tile == 0 && (double)ky * ky <= a.kf2) {
         double s = block_sum(fpart, red);
         if (tid == 0) a.fslots[iy] = s;
     }
@@ -148,28 +193,27 @@ __global__ void __launch_bounds__(N / EPT * W) pass1_kernel(P1Args a) {
     const long long oplane = (long long)N * NK * NZP;
     for (int f = f0; f < f1; ++f) {
         C v[EPT];
+        using R = T;
 #pragma unroll
         for (int m = 0; m < EPT; ++m) {
-            double2 r;
             if (f < 3) {
-                r = u[f][m];
+                v[m] = u[f][m];
             } else {
-                const double kx = (double)k_of_pos(t + TPF * m, N), kyd = (double)ky, kzd = (double)kz;
+                const R kx = (R)k_of_pos(t + TPF * m, N), kyd = (R)ky, kzd = (R)kz;
                 // w = i k x u
-                double xr, xi;
+                R xr, xi;
                 if (f == 3) { xr = kyd * u[2][m].x - kzd * u[1][m].x; xi = kyd * u[2][m].y - kzd * u[1][m].y; }
                 else if (f == 4) { xr = kzd * u[0][m].x - kx * u[2][m].x; xi = kzd * u[0][m].y - kx * u[2][m].y; }
                 else { xr = kx * u[1][m].x - kyd * u[0][m].x; xi = kx * u[1][m].y - kyd * u[0][m].y; }
-                r = make_double2(-xi, xr);
+                v[m] = cmk<C>(-xi, xr);
             }
-            v[m] = from_d2<T>(r);
         }
         fft<N, EPT, +1>(v, t, lds, w, W, tw);
         const int fo = f - f0;
 #pragma unroll
         for (int m = 0; m < EPT; ++m) {
             const int p = t + TPF * m;
-            out[fo * oplane + ((long long)p * NK + iy) * NZP + kz] = v[m];
+            out[fo * oplane + xidx<N>(p, iy, kz, NK, NZP)] = v[m];
         }
     }
 }
@@ -197,8 +241,9 @@ __global__ void __launch_bounds__(N / EPT * W) passy_kernel(PYArgs a) {
     const int x = blockIdx.y, f = blockIdx.z;
     const int kmax = a.kmax, NK = a.NK, NZP = a.NZP;
     const int rin = (DIR > 0) ? NK : N, rout = (DIR > 0) ? N : NK;
-    const C* in = (const C*)a.in + ((long long)f * N + x) * rin * NZP;
-    C* out = (C*)a.out + ((long long)f * N + x) * rout * NZP;
+    // DIR>0: in = A1 (x-tiled), out = A2 plain;  DIR<0: in = A3 plain, out = A4 (x-tiled)
+    const C* in = (const C*)a.in + (long long)f * N * rin * NZP;
+    C* out = (C*)a.out + (long long)f * N * rout * NZP;
     C v[EPT];
 #pragma unroll
     for (int m = 0; m < EPT; ++m) {
@@ -206,9 +251,9 @@ __global__ void __launch_bounds__(N / EPT * W) passy_kernel(PYArgs a) {
         if constexpr (DIR > 0) {
             const int k = k_of_pos(p, N);
             const bool ok = k >= -kmax && k <= kmax;
-            v[m] = ok ? in[(long long)cidx_of_k(k, NK) * NZP + kz] : cmk<C>(0, 0);
+            v[m] = ok ? in[xidx<N>(x, cidx_of_k(k, NK), kz, NK, NZP)] : cmk<C>(0, 0);
         } else {
-            v[m] = in[(long long)p * NZP + kz];
+            v[m] = in[((long long)x * N + p) * NZP + kz];
         }
     }
     fft<N, EPT, DIR>(v, t, lds, w, W, (const C*)a.twN);
@@ -216,10 +261,10 @@ __global__ void __launch_bounds__(N / EPT * W) passy_kernel(PYArgs a) {
     for (int m = 0; m < EPT; ++m) {
         const int p = t + TPF * m;
         if constexpr (DIR > 0) {
-            out[(long long)p * NZP + kz] = v[m];
+            out[((long long)x * N + p) * NZP + kz] = v[m];
         } else {
             const int k = k_of_pos(p, N);
-            if (k >= -kmax && k <= kmax) out[(long long)cidx_of_k(k, NK) * NZP + kz] = v[m];
+            if (k >= -kmax && k <= kmax) out[xidx<N>(x, cidx_of_k(k, NK), kz, NK, NZP)] = v[m];
         }
     }
 }
@@ -248,6 +293,9 @@ struct PZArgs {
     int track_max;
 };
 
+template <int M, int EPTZ> constexpr int zsync() { return (M / EPTZ <= 64 && 64 % (M / EPTZ) == 0) ? SYNC_WAVE : SYNC_BLOCK; }
+template <int M, int EPTZ> constexpr int zlds() { return fft_lds_len(M, EPTZ, true); }
+
 template <typename T, int N, int EPTZ>
 __device__ __forceinline__ void z_c2r_field(cpx_t<T> (&v)[EPTZ], const cpx_t<T>* line, int kmax, int t,
                                             cpx_t<T>* lds, int base, const cpx_t<T>* twM, const cpx_t<T>* twN) {
@@ -266,7 +314,7 @@ __device__ __forceinline__ void z_c2r_field(cpx_t<T> (&v)[EPTZ], const cpx_t<T>*
         e.y = -e.y;  // e^{+2 pi i k / N}
         v[m] = cadd(s, cmul_i<+1>(cmul(e, d)));
     }
-    fft<M, EPTZ, +1>(v, t, lds, base, 1, twM);
+    fft<M, EPTZ, +1, zsync<M, EPTZ>(), true>(v, t, lds, base, 1, twM);
 }
 
 template <typename T, int N, int EPTZ>
@@ -276,10 +324,11 @@ __device__ __forceinline__ void z_r2c_store(cpx_t<T> (&v)[EPTZ], cpx_t<T>* outli
     using C = cpx_t<T>;
     using R = decltype(C::x);
     constexpr int M = N / 2, TPF = M / EPTZ;
-    fft<M, EPTZ, -1>(v, t, lds, base, 1, twM);
+    constexpr int SY = zsync<M, EPTZ>();
+    fft<M, EPTZ, -1, SY, true>(v, t, lds, base, 1, twM);
 #pragma unroll
     for (int m = 0; m < EPTZ; ++m) lds[base + t + TPF * m] = v[m];
-    __syncthreads();
+    lds_sync<SY>();
 #pragma unroll
     for (int m = 0; m < EPTZ; ++m) {
         const int k = t + TPF * m;
@@ -299,7 +348,7 @@ __device__ __forceinline__ void z_r2c_store(cpx_t<T> (&v)[EPTZ], cpx_t<T>* outli
     }
     if (live)
         for (int k = M + t; k < NZP; k += TPF) outline[k] = cmk<C>(0, 0);
-    __syncthreads();
+    lds_sync<SY>();
 }
 
 template <typename T, int N, int EPTZ, int LPB, int MODE>
@@ -307,12 +356,13 @@ __global__ void __launch_bounds__(N / 2 / EPTZ * LPB) passz_kernel(PZArgs a) {
     using C = cpx_t<T>;
     using R = decltype(C::x);
     constexpr int M = N / 2, TPF = M / EPTZ;
-    __shared__ C lds[M * LPB];
+    constexpr int MP = zlds<M, EPTZ>();
+    __shared__ C lds[MP * LPB];
     const int tid = threadIdx.x, line = tid / TPF, t = tid % TPF;
     long long lid = (long long)blockIdx.x * LPB + line;
     const bool live = lid < (long long)N * N;
     if (!live) lid = (long long)N * N - 1;  // idle lines recompute the last line but never store
-    const int base = line * M;
+    const int base = line * MP;
     const int kmax = a.kmax, NZP = a.NZP;
     const C* twM = (const C*)a.twM;
     const C* twN = (const C*)a.twN;
@@ -346,7 +396,6 @@ __global__ void __launch_bounds__(N / 2 / EPTZ * LPB) passz_kernel(PZArgs a) {
             pr[1][m] = cmk<C>(uu[2][m].x * ww[0][m].x - uu[0][m].x * ww[2][m].x, uu[2][m].y * ww[0][m].y - uu[0][m].y * ww[2][m].y);
             pr[2][m] = cmk<C>(uu[0][m].x * ww[1][m].x - uu[1][m].x * ww[0][m].x, uu[0][m].y * ww[1][m].y - uu[1][m].y * ww[0][m].y);
         }
-        __syncthreads();
 #pragma unroll
         for (int c = 0; c < 3; ++c) z_r2c_store<T, N, EPTZ>(pr[c], out + c * fstride, kmax, NZP, t, lds, base, twM, twN, live);
     } else if constexpr (MODE == ZM_REAL) {
@@ -363,7 +412,6 @@ __global__ void __launch_bounds__(N / 2 / EPTZ * LPB) passz_kernel(PZArgs a) {
                     out[(2 * n + 1) * 3 + c] = (double)v[m].y * a.scale;
                 }
             }
-            __syncthreads();
         }
     } else {  // ZM_R2C
         const double* in = (const double*)a.in + lid * (long long)N * 3;
@@ -377,6 +425,172 @@ __global__ void __launch_bounds__(N / 2 / EPTZ * LPB) passz_kernel(PZArgs a) {
             }
             z_r2c_store<T, N, EPTZ>(v, out + c * fstride, kmax, NZP, t, lds, base, twM, twN, live);
         }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Pass 3, RHS form, persistent per-wavefront version.  Each wavefront owns 64/TPF lines at a
+// time and loops over line groups; all LDS exchanges are wavefront-local.  Per line: six
+// half-length c2r transforms (u then w), the product u x w accumulated field by field
+// (P_x = u_y w_z - u_z w_y, ...), then three r2c transforms.  The coefficients X[k] of field f+1
+// are loaded while field f is transformed; the partner X[M-k] comes from LDS.  Twiddle tables
+// live in LDS (one copy per block).
+// ------------------------------------------------------------------------------------------
+template <typename T, int N, int EPTZ>
+__device__ __forceinline__ void zr_load(cpx_t<T> (&x)[EPTZ], const cpx_t<T>* line, int kmax, int t) {
+    constexpr int TPF = (N / 2) / EPTZ;
+#pragma unroll
+    for (int m = 0; m < EPTZ; ++m) {
+        const int k = t + TPF * m;
+        x[m] = (k <= kmax) ? line[k] : cmk<cpx_t<T>>(0, 0);
+    }
+}
+
+// c2r of one field whose X[k] (k = t + TPF m) are in x; result z[n] = x[2n] + i x[2n+1] in v
+template <typename T, int N, int EPTZ, int SY>
+__device__ __forceinline__ void zr_c2r(cpx_t<T> (&v)[EPTZ], const cpx_t<T> (&x)[EPTZ], int t, cpx_t<T>* lds, int base,
+                                       const cpx_t<T>* twM, const cpx_t<T>* twN) {
+    using C = cpx_t<T>;
+    constexpr int M = N / 2, TPF = M / EPTZ;
+#pragma unroll
+    for (int m = 0; m < EPTZ; ++m) lds[base + t + TPF * m] = x[m];
+    lds_sync<SY>();
+#pragma unroll
+    for (int m = 0; m < EPTZ; ++m) {
+        const int k = t + TPF * m;
+        C xk = x[m];
+        C xr = lds[base + ((M - k) & (M - 1))];  // X[M-k]; for k = 0 this is X[0], replaced below
+        if (k == 0) { xk.y = 0; xr = cmk<C>(0, 0); }  // X[N/2] is outside the 2/3 box: zero
+        const C xrc = cconj(xr);
+        const C sm = cadd(xk, xrc), d = csub(xk, xrc);
+        C e = twN[k];
+        e.y = -e.y;  // e^{+2 pi i k / N}
+        v[m] = cadd(sm, cmul_i<+1>(cmul(e, d)));
+    }
+    lds_sync<SY>();
+    fft<M, EPTZ, +1, SY, true>(v, t, lds, base, 1, twM);
+}
+
+template <typename T, int N, int EPTZ, int SY>
+__device__ __forceinline__ void zr_r2c_store(cpx_t<T> (&v)[EPTZ], cpx_t<T>* outline, int kmax, int NZP, int t,
+                                             cpx_t<T>* lds, int base, const cpx_t<T>* twM, const cpx_t<T>* twN,
+                                             bool live) {
+    using C = cpx_t<T>;
+    using R = decltype(C::x);
+    constexpr int M = N / 2, TPF = M / EPTZ;
+    fft<M, EPTZ, -1, SY, true>(v, t, lds, base, 1, twM);
+#pragma unroll
+    for (int m = 0; m < EPTZ; ++m) lds[base + t + TPF * m] = v[m];
+    lds_sync<SY>();
+#pragma unroll
+    for (int m = 0; m < EPTZ; ++m) {
+        const int k = t + TPF * m;
+        if (live && k < NZP) {
+            C r = cmk<C>(0, 0);
+            if (k <= kmax) {
+                const C zk = v[m];
+                const C zr = cconj(lds[base + ((M - k) & (M - 1))]);
+                const C sm = cadd(zk, zr), d = csub(zk, zr);
+                const C ed = cmul(twN[k], d);
+                r = cmk<C>(R(0.5) * (sm.x + ed.y), R(0.5) * (sm.y - ed.x));
+            }
+            outline[k] = r;
+        }
+    }
+    if (live)
+        for (int k = M + t; k < NZP; k += TPF) outline[k] = cmk<C>(0, 0);
+    lds_sync<SY>();
+}
+
+template <typename T, int N, int EPTZ, int WPB, int OCC>
+__global__ void __launch_bounds__(64 * WPB, OCC) passz_rhs_kernel(PZArgs a) {
+    using C = cpx_t<T>;
+    constexpr int M = N / 2, TPF = M / EPTZ, LPW = 64 / TPF;
+    static_assert(TPF <= 64 && 64 % TPF == 0 && (M & (M - 1)) == 0, "line must fit a wavefront, M power of two");
+    constexpr int SY = SYNC_WAVE;
+    constexpr int MP = zlds<M, EPTZ>();
+    __shared__ C lds[WPB * LPW * MP];
+    __shared__ C twm_s[M];
+    __shared__ C twn_s[M];
+    for (int i = threadIdx.x; i < M; i += blockDim.x) {
+        twm_s[i] = ((const C*)a.twM)[i];
+        twn_s[i] = ((const C*)a.twN)[i];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int sub = lane / TPF, t = lane % TPF;
+    const int base = (wv * LPW + sub) * MP;
+    const int kmax = a.kmax, NZP = a.NZP;
+    const long long nlines = (long long)N * N;
+    const long long ngroups = (nlines + LPW - 1) / LPW;
+    const long long fstride = nlines * NZP;
+    const C* in0 = (const C*)a.in;
+    C* out0 = (C*)a.out;
+    double mx = 0.0;
+
+    long long g = (long long)blockIdx.x * WPB + wv;
+    C cur[EPTZ];
+    if (g < ngroups) {
+        long long lid = g * LPW + sub;
+        if (lid >= nlines) lid = nlines - 1;
+        zr_load<T, N, EPTZ>(cur, in0 + lid * NZP, kmax, t);
+    }
+    for (; g < ngroups; g += (long long)gridDim.x * WPB) {
+        long long lid = g * LPW + sub;
+        const bool live = lid < nlines;
+        if (!live) lid = nlines - 1;  // idle lanes recompute the last line but never store
+        const C* in = in0 + lid * NZP;
+        C ur[3][EPTZ], pr[3][EPTZ];
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+            C nxt[EPTZ];
+            if (f < 5) zr_load<T, N, EPTZ>(nxt, in + (f + 1) * fstride, kmax, t);
+            C v[EPTZ];
+            zr_c2r<T, N, EPTZ, SY>(v, cur, t, lds, base, twm_s, twn_s);
+#pragma unroll
+            for (int m = 0; m < EPTZ; ++m) {
+                const C w = v[m];
+                if (f < 3) {
+                    ur[f][m] = w;
+                } else if (f == 3) {  // w_x: P_y = u_z w_x, P_z = -u_y w_x
+                    pr[1][m] = cmk<C>(ur[2][m].x * w.x, ur[2][m].y * w.y);
+                    pr[2][m] = cmk<C>(-(ur[1][m].x * w.x), -(ur[1][m].y * w.y));
+                } else if (f == 4) {  // w_y: P_x = -u_z w_y, P_z += u_x w_y
+                    pr[0][m] = cmk<C>(-(ur[2][m].x * w.x), -(ur[2][m].y * w.y));
+                    pr[2][m] = cmk<C>(pr[2][m].x + ur[0][m].x * w.x, pr[2][m].y + ur[0][m].y * w.y);
+                } else {  // w_z: P_x += u_y w_z, P_y -= u_x w_z
+                    pr[0][m] = cmk<C>(pr[0][m].x + ur[1][m].x * w.x, pr[0][m].y + ur[1][m].y * w.y);
+                    pr[1][m] = cmk<C>(pr[1][m].x - ur[0][m].x * w.x, pr[1][m].y - ur[0][m].y * w.y);
+                }
+            }
+            if (f < 5) {
+#pragma unroll
+                for (int m = 0; m < EPTZ; ++m) cur[m] = nxt[m];
+            }
+        }
+        // prefetch field 0 of the next line group while the products are transformed back
+        const long long gn = g + (long long)gridDim.x * WPB;
+        if (gn < ngroups) {
+            long long ln = gn * LPW + sub;
+            if (ln >= nlines) ln = nlines - 1;
+            zr_load<T, N, EPTZ>(cur, in0 + ln * NZP, kmax, t);
+        }
+        if (a.track_max && live) {
+#pragma unroll
+            for (int m = 0; m < EPTZ; ++m) {
+                const double s0 = fabs((double)ur[0][m].x) + fabs((double)ur[1][m].x) + fabs((double)ur[2][m].x);
+                const double s1 = fabs((double)ur[0][m].y) + fabs((double)ur[1][m].y) + fabs((double)ur[2][m].y);
+                mx = fmax(mx, fmax(s0, s1));
+            }
+        }
+        C* out = out0 + lid * NZP;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) zr_r2c_store<T, N, EPTZ, SY>(pr[c], out + c * fstride, kmax, NZP, t, lds, base, twm_s, twn_s, live);
+    }
+    if (a.track_max) {
+        mx *= a.scale;
+        for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+        if (lane == 0) atomicMax(&a.sc->maxbits, (unsigned long long)__double_as_longlong(mx));
     }
 }
 
@@ -408,10 +622,12 @@ template <typename T, int N, int EPT, int W, int MODE>
 __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
     using C = cpx_t<T>;
     constexpr int TPF = N / EPT;
+    constexpr bool PAIR = W * sizeof(C) < 128;
     __shared__ C lds[N * W];
+    int tile, iy;
+    if (!tile_of_block<PAIR>(a.NZP / W, a.NK, tile, iy)) return;
     const int tid = threadIdx.x, w = tid % W, t = tid / W;
-    const int kz = blockIdx.x * W + w;
-    const int iy = blockIdx.y;
+    const int kz = tile * W + w;
     const int kmax = a.kmax, NK = a.NK, NZP = a.NZP;
     const int ky = k_of_cidx(iy, kmax, NK);
     const long long plane = (long long)NK * NK * NZP;
@@ -419,15 +635,12 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
     const C* in = (const C*)a.in;
     const C* tw = (const C*)a.twN;
 
-    double2 r[3][EPT];
+    C r[3][EPT];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        C v[EPT];
 #pragma unroll
-        for (int m = 0; m < EPT; ++m) v[m] = in[c * iplane + ((long long)(t + TPF * m) * NK + iy) * NZP + kz];
-        fft<N, EPT, -1>(v, t, lds, w, W, tw);
-#pragma unroll
-        for (int m = 0; m < EPT; ++m) r[c][m] = to_d2<T>(v[m]);
+        for (int m = 0; m < EPT; ++m) r[c][m] = in[c * iplane + xidx<N>(t + TPF * m, iy, kz, NK, NZP)];
+        fft<N, EPT, -1>(r[c], t, lds, w, W, tw);
     }
 
     double ffac = 0.0;
@@ -449,20 +662,20 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
         const int p = t + TPF * m;
         const int kx = k_of_pos(p, N);
         if (kx < -kmax || kx > kmax || kz >= NZP) continue;
-        const long long idx = ((long long)cidx_of_k(kx, NK) * NK + iy) * NZP + kz;
+        const long long idx = sidx(cidx_of_k(kx, NK), iy, kz, NK, NZP);
         const long long k2 = (long long)kx * kx + (long long)ky * ky + (long long)kz * kz;
         const bool keep = k2 <= kcut2 && kz <= kmax;
         double2 nl[3];
         if (MODE == EP_STORE) {
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-                nl[c] = keep ? make_double2(r[c][m].x * a.norm, r[c][m].y * a.norm) : make_double2(0.0, 0.0);
+                nl[c] = keep ? make_double2((double)r[c][m].x * a.norm, (double)r[c][m].y * a.norm) : make_double2(0.0, 0.0);
 #pragma unroll
             for (int c = 0; c < 3; ++c) a.uh[c * plane + idx] = nl[c];
             continue;
         }
 #pragma unroll
-        for (int c = 0; c < 3; ++c) nl[c] = make_double2(r[c][m].x * a.norm, r[c][m].y * a.norm);
+        for (int c = 0; c < 3; ++c) nl[c] = make_double2((double)r[c][m].x * a.norm, (double)r[c][m].y * a.norm);
         if (k2 != 0 && keep) {
             const double kinv = 1.0 / (double)k2;
             const double kxd = kx, kyd = ky, kzd = kz;
@@ -561,8 +774,8 @@ __global__ void aos_to_state_kernel(const double2* __restrict__ aos, double2* __
     const long long kcut2 = (long long)kmax * kmax;
     for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < total; n += (long long)gridDim.x * blockDim.x) {
         const int kz = (int)(n % NZP);
-        const int iy = (int)((n / NZP) % NK);
-        const int ix = (int)(n / ((long long)NZP * NK));
+        const int ix = (int)((n / NZP) % NK);
+        const int iy = (int)(n / ((long long)NZP * NK));
         const int kx = k_of_cidx(ix, kmax, NK), ky = k_of_cidx(iy, kmax, NK);
         const long long k2 = (long long)kx * kx + (long long)ky * ky + (long long)kz * kz;
         const bool keep = kz <= kmax && k2 <= kcut2;
@@ -584,7 +797,7 @@ __global__ void compact_to_aos_kernel(const CT* __restrict__ src, double2* __res
         const int i = (int)(n / ((long long)Nzf * N));
         const int kx = k_of_pos(i, N), ky = k_of_pos(j, N);
         const bool in = kz <= kmax && kx >= -kmax && kx <= kmax && ky >= -kmax && ky <= kmax;
-        const long long s = ((long long)cidx_of_k(kx, NK) * NK + cidx_of_k(ky, NK)) * NZP + kz;
+        const long long s = sidx(cidx_of_k(kx, NK), cidx_of_k(ky, NK), kz, NK, NZP);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             double2 v = make_double2(0.0, 0.0);
@@ -626,7 +839,7 @@ __global__ void measures_kernel(const double2* __restrict__ uh, double* __restri
         long long zhi = isqrt_ll(hi - kh2);
         if (zhi > kmax) zhi = kmax;
         for (long long kz = zlo; kz <= zhi; ++kz) {
-            const long long idx = ((long long)ix * NK + iy) * NZP + kz;
+            const long long idx = sidx(ix, iy, kz, NK, NZP);
             const double kxd = (double)kx, kyd = (double)ky, kzd = (double)kz;
             double2 u[3], w[3], c[3];
             for (int d = 0; d < 3; ++d) u[d] = uh[d * plane + idx];
@@ -766,7 +979,7 @@ __global__ void ic_k4_kernel(double2* __restrict__ uh, double* __restrict__ part
     const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     double e = 0.0;
     if (n < plane) {
-        const int kz = (int)(n % NZP), iy = (int)((n / NZP) % NK), ix = (int)(n / ((long long)NZP * NK));
+        const int kz = (int)(n % NZP), ix = (int)((n / NZP) % NK), iy = (int)(n / ((long long)NZP * NK));
         const long long kx = k_of_cidx(ix, kmax, NK), ky = k_of_cidx(iy, kmax, NK);
         double2 v[3];
         if (kz > kmax) {

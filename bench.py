@@ -1,0 +1,165 @@
+#!/usr/bin/env python3
+"""Benchmark: RK4 steps/s of the forced 512^3 box, FP64 vs FP32-bulk, on one MI355X.
+
+Workload (the 512^3 forced stationary turbulence configuration): periodic [0,2pi)^3, N=512,
+nu=1e-3, constant-power forcing eps_in=0.1 on 0<|k|<=2.5, RK4 with the reference's Crank-Nicolson
+viscous factor, CFL 0.5 adaptive step computed on the device, random-phase k^4 exp(-2(k/2)^2)
+start (Philox seed 2, 1/2<|u|^2> = 0.5).  The state is resident in HBM before timing starts.
+
+Prints one JSON line.  `value` is FP32-bulk steps/s; the FP64 plain path is measured the same way
+and reported under "fp64".  `roofline` describes the kernel with the largest share of step time,
+timed with HIP events on the library's own stream.  `cpu_baseline` times the FP64 CPU oracle
+(a C restatement of the reference algorithm, OpenMP) on a bounded sample: one nonlinear-term
+evaluation at 512^3, scaled to steps/s as 1/(4 x that time).
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+HBM_PEAK_GBS = 8000.0
+METRIC = "RK4 steps/s at 512^3 FP64 vs FP32-bulk, % HBM peak; closure-LES speedup"
+PASS_NAMES = ["pass1_invx_curl", "pass2_invy", "pass3_z_c2r_uxw_r2c", "pass4_fwdy", "pass5_fwdx_epilogue"]
+
+
+def load_nsg():
+    pkg = os.path.join(ROOT, "3d_navier_stokes_amd")
+    spec = importlib.util.spec_from_file_location("navier_stokes_amd", os.path.join(pkg, "__init__.py"),
+                                                  submodule_search_locations=[pkg])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["navier_stokes_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_mode(nsg, N, precision, steps, warmup, prof_iters):
+    s = nsg.Solver(N, nu=1e-3, dt=1e-3, scheme="rk4cn", precision=precision, cfl=0.5, force_eps=0.1,
+                   force_kf=2.5)
+    s.init("k4", k0=2.0, energy=0.5, seed=2)
+    log(f"[bench] {precision}: warmup {warmup} steps")
+    s.step(warmup)
+    t0 = time.perf_counter()
+    s.step(steps)  # returns after the stream has drained
+    el = time.perf_counter() - t0
+    ms, by = s.profile(prof_iters)
+    log(f"[bench] {precision}: {steps / el:.2f} steps/s; pass ms {['%.3f' % m for m in ms[:5]]}")
+    passes = {}
+    for i, name in enumerate(PASS_NAMES):
+        gbs = by[i] / (ms[i] * 1e-3) / 1e9 if ms[i] > 0 else 0.0
+        passes[name] = {"ms": ms[i], "bytes": by[i], "GBps": gbs, "frac": gbs / HBM_PEAK_GBS}
+    dom = max(range(5), key=lambda i: ms[i])
+    achieved = by[dom] / (ms[dom] * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": PASS_NAMES[dom], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_launch": by[dom], "ms_per_launch": ms[dom]}
+    step_gb = s.step_bytes() / 1e9
+    out = {"steps_per_s": steps / el, "ms_per_step": 1e3 * el / steps, "roofline": roof, "passes": passes,
+           "step_bytes_GB": step_gb, "step_GBps": step_gb / (el / steps)}
+    return s, out
+
+
+def cpu_baseline(N):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O  # the checker; timed here only as the reported CPU baseline
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    O.set_threads(threads)
+    uh = O.ic_k4(N, k0=2.0, energy=0.5, seed=2)
+    t0 = time.perf_counter()
+    O.rhs(uh, force_eps=0.1, force_kf=2.5)
+    el = time.perf_counter() - t0
+    return {"value": 1.0 / (4.0 * el), "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"one nonlinear-term evaluation (9 FP64 3-D real FFTs + products) at {N}^3 took {el:.2f} s; "
+                      f"steps/s = 1/(4 x that), stage combinations not counted"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--prof-iters", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=512)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs one box on one GPU; --gpus N>1 is not supported "
+                                   "(multi-GPU slab decomposition is a design note only)"}))
+        sys.exit(2)
+
+    nsg = load_nsg()
+    N = args.n
+    s64, r64 = run_mode(nsg, N, "fp64", args.steps, args.warmup, args.prof_iters)
+    s32, r32 = run_mode(nsg, N, "fp32bulk", args.steps, args.warmup, args.prof_iters)
+
+    # FP32-bulk error vs FP64 from one identical state: one step each, energy spectrum and totals
+    s32.set_state(s64.get_state())
+    t64, dt64, _ = s64.time()
+    s32.set_time(t64, dt64)
+    s64.step(1)
+    s32.step(1)
+    m64, ek64, _ = s64.measures()
+    m32, ek32, _ = s32.measures()
+    sel = ek64 > 1e-300
+    ek_err = float(np.max(np.abs(ek32[sel] - ek64[sel]) / ek64[sel]))
+    kc = N // 3
+    ek_err_kc = float(np.max(np.abs(ek32[1:kc + 1] - ek64[1:kc + 1]) / ek64[1:kc + 1]))
+    u64 = s64.get_state()
+    u32 = s32.get_state()
+    state_err = float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64))
+    s64.close()
+    s32.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log(f"[bench] cpu baseline at {args.cpu_n}^3 ...")
+        cpu = cpu_baseline(args.cpu_n)
+        log(f"[bench] cpu baseline {cpu['value']:.4f} steps/s")
+
+    line = {
+        "metric": METRIC,
+        "value": round(r32["steps_per_s"], 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r32["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32-bulk (f32 FFTs/products, f64 state/update/reductions)",
+        "data": "synthetic",
+        "config": {"workload": f"forced isotropic turbulence {N}^3, nu=1e-3, constant-power forcing eps_in=0.1 "
+                               f"on |k|<=2.5, RK4 (reference RK4-CN), CFL 0.5, k^4 random-phase start seed 2",
+                   "N": N, "scheme": "rk4cn", "parallelism": "single box, one GPU"},
+        "fp64": {"value": round(r64["steps_per_s"], 3), "ms_per_step": round(r64["ms_per_step"], 3),
+                 "roofline": r64["roofline"], "passes": r64["passes"], "step_bytes_GB": r64["step_bytes_GB"],
+                 "step_GBps": r64["step_GBps"]},
+        "fp32bulk_over_fp64": round(r32["steps_per_s"] / r64["steps_per_s"], 3),
+        "fp32bulk_error_vs_fp64_one_step": {"E_k_max_rel": ek_err, "E_k_max_rel_k_le_kc": ek_err_kc,
+                                            "energy_rel": abs(m32.energy - m64.energy) / m64.energy,
+                                            "dissipation_rel": abs(m32.dissipation - m64.dissipation) / m64.dissipation,
+                                            "state_rel_l2": state_err},
+        "roofline": r32["roofline"],
+        "passes": r32["passes"],
+        "step_bytes_GB": r32["step_bytes_GB"],
+        "step_GBps": r32["step_GBps"],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()
