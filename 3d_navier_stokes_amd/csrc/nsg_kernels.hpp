@@ -77,6 +77,9 @@ __device__ __forceinline__ void stage_in_coef(int scheme, int stage, double D, d
 template <typename T> __device__ __forceinline__ double2 to_d2(cpx_t<T> v) { return make_double2((double)v.x, (double)v.y); }
 template <typename T> __device__ __forceinline__ cpx_t<T> from_d2(double2 v) { return cmk<cpx_t<T>>((T)v.x, (T)v.y); }
 
+// max that propagates NaN (a NaN state must reach the host's non-finite check)
+__device__ __forceinline__ double nanmax(double a, double b) { return (a >= b || a != a) ? a : b; }
+
 // deterministic block sum (fixed tree); red must hold blockDim.x doubles
 __device__ __forceinline__ double block_sum(double v, double* red) {
     const int tid = threadIdx.x, n = blockDim.x;
@@ -382,10 +385,10 @@ __global__ void __launch_bounds__(N / 2 / EPTZ * LPB) passz_kernel(PZArgs a) {
             for (int m = 0; m < EPTZ; ++m) {
                 double s0 = fabs((double)uu[0][m].x) + fabs((double)uu[1][m].x) + fabs((double)uu[2][m].x);
                 double s1 = fabs((double)uu[0][m].y) + fabs((double)uu[1][m].y) + fabs((double)uu[2][m].y);
-                mx = fmax(mx, fmax(s0, s1));
+                mx = nanmax(mx, nanmax(s0, s1));
             }
             mx *= a.scale;
-            for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+            for (int off = 32; off > 0; off >>= 1) mx = nanmax(mx, __shfl_xor(mx, off));
             if ((tid & 63) == 0 && live) atomicMax(&a.sc->maxbits, (unsigned long long)__double_as_longlong(mx));
         }
         C pr[3][EPTZ];
@@ -580,7 +583,7 @@ __global__ void __launch_bounds__(64 * WPB, OCC) passz_rhs_kernel(PZArgs a) {
             for (int m = 0; m < EPTZ; ++m) {
                 const double s0 = fabs((double)ur[0][m].x) + fabs((double)ur[1][m].x) + fabs((double)ur[2][m].x);
                 const double s1 = fabs((double)ur[0][m].y) + fabs((double)ur[1][m].y) + fabs((double)ur[2][m].y);
-                mx = fmax(mx, fmax(s0, s1));
+                mx = nanmax(mx, nanmax(s0, s1));
             }
         }
         C* out = out0 + lid * NZP;
@@ -589,7 +592,7 @@ __global__ void __launch_bounds__(64 * WPB, OCC) passz_rhs_kernel(PZArgs a) {
     }
     if (a.track_max) {
         mx *= a.scale;
-        for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+        for (int off = 32; off > 0; off >>= 1) mx = nanmax(mx, __shfl_xor(mx, off));
         if (lane == 0) atomicMax(&a.sc->maxbits, (unsigned long long)__double_as_longlong(mx));
     }
 }

@@ -1,0 +1,83 @@
+"""Full-size checks at 256^3 and 512^3: parity at 256^3 against the oracle, size-independent
+properties at 512^3 (divergence, dealias support, Hermitian plane, energy conservation of the
+nonlinear term, exact Beltrami decay, FP32-bulk error from identical states, determinism)."""
+import numpy as np
+import pytest
+
+from conftest import rel, wavenumbers
+
+pytestmark = pytest.mark.gpu
+
+
+def test_rhs_parity_256(nsg, oracle):
+    N = 256
+    uh = oracle.ic_k4(N, 2.0, 0.5, 5)
+    with nsg.Solver(N) as s:
+        assert rel(s.rhs(uh), oracle.rhs(uh)) < 1e-12
+
+
+def test_shapiro_exact_decay_256(nsg):
+    N, nu, dt = 256, 1e-3, 1e-3
+    with nsg.Solver(N, nu=nu, dt=dt) as s:
+        s.init("shapiro")
+        u0 = s.get_state()
+        s.step(3)
+        D = nu * 12 * dt
+        assert rel(s.get_state(), u0 * ((2 - D) / (2 + D)) ** 3) < 1e-12
+
+
+def _half_weights(N):
+    KX, KY, KZ = wavenumbers(N)
+    return np.where((KZ == 0) | (KZ == N // 2), 1.0, 2.0), (KX, KY, KZ)
+
+
+def test_rhs_properties_512(nsg):
+    N = 512
+    with nsg.Solver(N, precision="fp64") as s:
+        s.init("k4", k0=2.0, energy=0.5, seed=2)
+        uh = s.get_state()
+        nl = s.rhs(uh)
+    w, (KX, KY, KZ) = _half_weights(N)
+    scale = np.max(np.abs(nl))
+    div = KX * nl[..., 0] + KY * nl[..., 1] + KZ * nl[..., 2]
+    assert np.max(np.abs(div)) < 1e-12 * scale * N
+    k2 = KX * KX + KY * KY + KZ * KZ
+    assert not np.any(nl[k2 > (N // 3) ** 2])
+    p = nl[:, :, 0, 0]
+    m = np.conj(p[(-np.arange(N)) % N][:, (-np.arange(N)) % N])
+    assert np.max(np.abs(p - m)) < 1e-12 * scale
+    tr = np.sum(w[..., None] * np.real(np.conj(uh) * nl))
+    assert abs(tr) < 1e-11 * np.sum(w[..., None] * np.abs(uh) * np.abs(nl))
+
+
+def test_fp32bulk_vs_fp64_512(nsg):
+    """one step from an identical forced-turbulence state: E(k) within 1e-5 of FP64 on every shell
+    holding at least 1e-10 of the peak shell's energy (below that FP32 round-off is a noise floor)"""
+    N = 512
+    kw = dict(nu=1e-3, dt=1e-3, force_eps=0.1)
+    with nsg.Solver(N, **kw) as a, nsg.Solver(N, precision="fp32bulk", **kw) as b:
+        a.init("k4", k0=2.0, energy=0.5, seed=2)
+        b.set_state(a.get_state())
+        a.step(1)
+        b.step(1)
+        ta, ea, _ = a.measures()
+        tb, eb, _ = b.measures()
+    sel = ea >= 1e-10 * ea.max()
+    assert np.max(np.abs(eb[sel] - ea[sel]) / ea[sel]) < 1e-5
+    assert abs(tb.energy - ta.energy) < 1e-7 * ta.energy
+    assert abs(tb.dissipation - ta.dissipation) < 1e-6 * ta.dissipation
+
+
+def test_determinism_512(nsg):
+    """two runs from the same state are bitwise identical (no order-dependent reductions)"""
+    N = 512
+    out = []
+    for _ in range(2):
+        with nsg.Solver(N, nu=1e-3, dt=1e-3, cfl=0.5, force_eps=0.1) as s:
+            s.init("k4", k0=2.0, energy=0.5, seed=3)
+            s.step(2)
+            tot, ek, _ = s.measures()
+            out.append((s.get_state(), s.time(), np.array(tot), ek))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]
+    assert np.array_equal(out[0][2], out[1][2]) and np.array_equal(out[0][3], out[1][3])
