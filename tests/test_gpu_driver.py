@@ -1,0 +1,70 @@
+"""End-to-end host driver on the GPU: the Taylor-Green configuration (64^3, nu=1/1600, dt=1e-3,
+T=0.2, saves every 10 steps, Solver/src/solver.c:50-208 loop semantics), its HDF5 output against
+the oracle, and a restart (-z) that continues bitwise."""
+import glob
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import h5read
+from conftest import ROOT, rel
+
+pytestmark = pytest.mark.gpu
+SOLVER = os.path.join(ROOT, "3d_navier_stokes_amd", "host", "nsg_solver")
+NU = 1.0 / 1600.0
+
+
+def run(args, tmp):
+    p = subprocess.run([SOLVER, "-o", str(tmp) + "/"] + args, capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    return p.stdout
+
+
+def main_file(tmp, tag):
+    return glob.glob(os.path.join(str(tmp), f"SIM_DATA_*TAG[[]{tag}[]]", "Main_HDF_Data.h5"))[0]
+
+
+def test_taylor_green_run(oracle, tmp_path):
+    out = run(["-n", "64", "-n", "64", "-n", "64", "-s", "0", "-e", "0.2", "-h", "1e-3", "-v", str(NU), "-i",
+               "TAYLOR_GREEN", "-p", "10", "-t", "TG", "--no-realspace"], tmp_path)
+    assert "Total Iters: 200\t Saving Iters: 21" in out
+    with h5read.File(main_file(tmp_path, "TG")) as f:
+        iters = sorted(n for n in f.list("/") if n.startswith("Iter_"))
+        assert iters == [f"Iter_{i:05d}" for i in range(21)]
+        t = f.read("/Time")
+        assert np.allclose(t, 0.01 * np.arange(21), rtol=0, atol=1e-12)
+        E = f.read("/TotalEnergy")
+        assert abs(E[0] - np.pi ** 3) < 1e-12 * np.pi ** 3
+        assert np.all(np.diff(E) < 0)  # viscous decay, no forcing
+        u = oracle.ic_taylor_green(64)
+        u = oracle.step(u, NU, 1e-3, nsteps=20)
+        assert rel(f.read("/Iter_00002/u_hat", "complex"), u) < 1e-12
+        to, _, _ = oracle.measures(u, NU)
+        assert abs(E[2] - to[0]) < 1e-12 * to[0]
+        assert abs(f.read("/EnergyDissipation")[2] - to[4]) < 1e-12 * to[4]
+        assert abs(f.attr("/Iter_00002", "TimeValue") - 0.02) < 1e-15
+
+
+def test_restart_continues(tmp_path):
+    base = ["-n", "32", "-n", "32", "-n", "32", "-h", "1e-3", "-v", "0.01", "-i", "K4", "--k0", "2", "--seed", "5",
+            "-p", "5", "--no-realspace", "-f", "CONST_POWER", "--scheme", "ifrk4"]
+    run(base + ["-e", "0.02", "-t", "FULL"], tmp_path)
+    run(base + ["-e", "0.01", "-t", "HALF"], tmp_path)
+    run(base + ["-s", "0.01", "-e", "0.02", "-t", "REST", "-z", main_file(tmp_path, "HALF")], tmp_path)
+    with h5read.File(main_file(tmp_path, "FULL")) as a, h5read.File(main_file(tmp_path, "REST")) as b:
+        ua = a.read("/Iter_00004/u_hat", "complex")  # t = 0.02
+        ub = b.read("/Iter_00002/u_hat", "complex")  # restart at t = 0.01 (Iter_00000) + 10 steps
+        assert abs(b.attr("/Iter_00002", "TimeValue") - 0.02) < 1e-15
+        assert np.array_equal(ua, ub)
+
+
+def test_realspace_datasets(oracle, tmp_path):
+    run(["-n", "16", "-n", "16", "-n", "16", "-e", "0.002", "-h", "1e-3", "-v", "0.01", "-i", "TAYLOR_GREEN", "-p",
+         "1", "-t", "RS"], tmp_path)
+    with h5read.File(main_file(tmp_path, "RS")) as f:
+        uh = f.read("/Iter_00001/u_hat", "complex")
+        assert rel(f.read("/Iter_00001/u"), oracle.realspace(uh, 0)) < 1e-12
+        assert rel(f.read("/Iter_00001/w"), oracle.realspace(uh, 1)) < 1e-12
+        assert rel(f.read("/Iter_00001/w_hat", "complex"), oracle.curl(uh)) < 1e-14
