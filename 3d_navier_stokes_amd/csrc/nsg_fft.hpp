@@ -224,4 +224,88 @@ __device__ __forceinline__ void fft(C (&v)[EPT], int t, C* lds, int base, int PS
     fft_stages<L, EPT, TPF, 0, NS, DIR, SYNC, PAD>(v, t, lds, base, PS, tw);
 }
 
+// ------------------------------------------------------------------------------------------
+// Register-twiddle variant for transforms whose threads keep the same t across many transforms
+// (the persistent z pass): the base twiddle W_{PR}^{b mod P} of every butterfly of stages >= 1 is
+// loaded once (fft_load_base_twiddles) and its powers q = 2..R-1 are formed by multiplication.
+// ------------------------------------------------------------------------------------------
+constexpr int num_base_twiddles(int L, int ept) {
+    int n = 0;
+    for (int s = 1; s < num_stages(L, ept); ++s) n += ept / stage_radix(L, ept, s);
+    return n;
+}
+template <int L, int EPT, int S> constexpr int base_twiddle_offset() {
+    int n = 0;
+    for (int s = 1; s < S; ++s) n += EPT / stage_radix(L, EPT, s);
+    return n;
+}
+
+template <int L, int EPT, int TPF, int S, int NTW, typename C>
+__device__ __forceinline__ void load_base_stage(C (&w)[NTW], int t, const C* __restrict__ tw) {
+    constexpr int NS = num_stages(L, EPT);
+    if constexpr (S < NS) {
+        constexpr int R = stage_radix(L, EPT, S);
+        constexpr int P = stage_p(L, EPT, S);
+        constexpr int B = EPT / R;
+        constexpr int off = base_twiddle_offset<L, EPT, S>();
+#pragma unroll
+        for (int beta = 0; beta < B; ++beta) w[off + beta] = tw[((t + TPF * beta) % P) * (L / (P * R))];
+        load_base_stage<L, EPT, TPF, S + 1, NTW>(w, t, tw);
+    }
+}
+// w[] receives exp(-2 pi i (b mod P)/(P R)) for every butterfly of stages 1..NS-1 (forward sign)
+template <int L, int EPT, typename C>
+__device__ __forceinline__ void fft_load_base_twiddles(C (&w)[num_base_twiddles(L, EPT) > 0 ? num_base_twiddles(L, EPT) : 1],
+                                                       int t, const C* __restrict__ tw) {
+    constexpr int TPF = L / EPT;
+    load_base_stage<L, EPT, TPF, 1, (num_base_twiddles(L, EPT) > 0 ? num_base_twiddles(L, EPT) : 1)>(w, t, tw);
+}
+
+template <int L, int EPT, int TPF, int S, int DIR, int NTW, typename C>
+__device__ __forceinline__ void stage_compute_reg(C (&v)[EPT], const C (&w)[NTW]) {
+    constexpr int R = stage_radix(L, EPT, S);
+    constexpr int B = EPT / R;
+    constexpr int off = (S > 0) ? base_twiddle_offset<L, EPT, S>() : 0;
+#pragma unroll
+    for (int beta = 0; beta < B; ++beta) {
+        if constexpr (S > 0) {
+            C w1 = w[off + beta];
+            if constexpr (DIR > 0) w1.y = -w1.y;
+            C wq = w1;
+#pragma unroll
+            for (int q = 1; q < R; ++q) {
+                v[beta + B * q] = cmul(v[beta + B * q], wq);
+                if (q + 1 < R) wq = cmul(wq, w1);
+            }
+        }
+        if constexpr (R == 2) dft2<DIR>(v[beta], v[beta + B]);
+        else if constexpr (R == 3) dft3<DIR>(v[beta], v[beta + B], v[beta + 2 * B]);
+        else if constexpr (R == 4) dft4<DIR>(v[beta], v[beta + B], v[beta + 2 * B], v[beta + 3 * B]);
+        else if constexpr (R == 8)
+            dft8<DIR>(v[beta], v[beta + B], v[beta + 2 * B], v[beta + 3 * B], v[beta + 4 * B], v[beta + 5 * B],
+                      v[beta + 6 * B], v[beta + 7 * B]);
+    }
+}
+
+template <int L, int EPT, int TPF, int S, int NS, int DIR, int SYNC, bool PAD, int NTW, typename C>
+__device__ __forceinline__ void fft_stages_reg(C (&v)[EPT], int t, C* lds, int base, int PS, const C (&w)[NTW]) {
+    if constexpr (S < NS) {
+        stage_compute_reg<L, EPT, TPF, S, DIR, NTW>(v, w);
+        if constexpr (S + 1 < NS) {
+            stage_scatter<L, EPT, TPF, S, PAD>(v, t, lds, base, PS);
+            lds_sync<SYNC>();
+            gather_pattern<L, EPT, TPF, S, PAD>(v, t, lds, base, PS);
+            lds_sync<SYNC>();
+        }
+        fft_stages_reg<L, EPT, TPF, S + 1, NS, DIR, SYNC, PAD, NTW>(v, t, lds, base, PS, w);
+    }
+}
+
+template <int L, int EPT, int DIR, int SYNC, bool PAD, int NTW, typename C>
+__device__ __forceinline__ void fft_reg(C (&v)[EPT], int t, C* lds, int base, int PS, const C (&w)[NTW]) {
+    static_assert(schedule_ok(L, EPT), "unsupported FFT length / elements-per-thread");
+    constexpr int TPF = L / EPT;
+    fft_stages_reg<L, EPT, TPF, 0, num_stages(L, EPT), DIR, SYNC, PAD, NTW>(v, t, lds, base, PS, w);
+}
+
 }  // namespace nsg

@@ -31,11 +31,12 @@ struct DevScalars {
 // ------------------------------------------------------------------------------------------
 // layouts
 //   state / K_s : [3][NK (ky)][NK (kx)][NZP]  -- y-major, so an x pass at fixed ky stays inside
-//                 one NK*NZP plane
-//   A1 / A4     : [F][N/XB][NK][XB][NZP]      -- x rows tiled in blocks of XB: an x pass touches
-//                 N/XB page runs instead of N, a y pass strides XB*NZP elements
+//                 one NK*NZP plane (x-major state rows 1 MB apart thrash the TLB: 5e6 misses
+//                 per launch at 512^3)
+//   A1 / A4     : [F][N][NK][NZP]             -- x-major; measured faster than x-row tiles of
+//                 4..32 rows or the y-major order, which slow the y passes more than they help x
 // ------------------------------------------------------------------------------------------
-template <int N> constexpr int xtile() { return (N % 16 == 0) ? 16 : 4; }
+template <int N> constexpr int xtile() { return 1; }
 __device__ __forceinline__ long long sidx(int ix, int iy, int kz, int NK, int NZP) {
     return ((long long)iy * NK + ix) * NZP + kz;
 }
@@ -450,9 +451,11 @@ __device__ __forceinline__ void zr_load(cpx_t<T> (&x)[EPTZ], const cpx_t<T>* lin
 }
 
 // c2r of one field whose X[k] (k = t + TPF m) are in x; result z[n] = x[2n] + i x[2n+1] in v
+template <int M, int EPTZ> constexpr int zntw() { return num_base_twiddles(M, EPTZ) > 0 ? num_base_twiddles(M, EPTZ) : 1; }
+
 template <typename T, int N, int EPTZ, int SY>
 __device__ __forceinline__ void zr_c2r(cpx_t<T> (&v)[EPTZ], const cpx_t<T> (&x)[EPTZ], int t, cpx_t<T>* lds, int base,
-                                       const cpx_t<T>* twM, const cpx_t<T>* twN) {
+                                       const cpx_t<T> (&wz)[zntw<N / 2, EPTZ>()], const cpx_t<T> (&ek)[EPTZ]) {
     using C = cpx_t<T>;
     constexpr int M = N / 2, TPF = M / EPTZ;
 #pragma unroll
@@ -466,22 +469,21 @@ __device__ __forceinline__ void zr_c2r(cpx_t<T> (&v)[EPTZ], const cpx_t<T> (&x)[
         if (k == 0) { xk.y = 0; xr = cmk<C>(0, 0); }  // X[N/2] is outside the 2/3 box: zero
         const C xrc = cconj(xr);
         const C sm = cadd(xk, xrc), d = csub(xk, xrc);
-        C e = twN[k];
-        e.y = -e.y;  // e^{+2 pi i k / N}
+        const C e = cconj(ek[m]);  // e^{+2 pi i k / N}
         v[m] = cadd(sm, cmul_i<+1>(cmul(e, d)));
     }
     lds_sync<SY>();
-    fft<M, EPTZ, +1, SY, true>(v, t, lds, base, 1, twM);
+    fft_reg<M, EPTZ, +1, SY, true>(v, t, lds, base, 1, wz);
 }
 
 template <typename T, int N, int EPTZ, int SY>
 __device__ __forceinline__ void zr_r2c_store(cpx_t<T> (&v)[EPTZ], cpx_t<T>* outline, int kmax, int NZP, int t,
-                                             cpx_t<T>* lds, int base, const cpx_t<T>* twM, const cpx_t<T>* twN,
-                                             bool live) {
+                                             cpx_t<T>* lds, int base, const cpx_t<T> (&wz)[zntw<N / 2, EPTZ>()],
+                                             const cpx_t<T> (&ek)[EPTZ], bool live) {
     using C = cpx_t<T>;
     using R = decltype(C::x);
     constexpr int M = N / 2, TPF = M / EPTZ;
-    fft<M, EPTZ, -1, SY, true>(v, t, lds, base, 1, twM);
+    fft_reg<M, EPTZ, -1, SY, true>(v, t, lds, base, 1, wz);
 #pragma unroll
     for (int m = 0; m < EPTZ; ++m) lds[base + t + TPF * m] = v[m];
     lds_sync<SY>();
@@ -494,7 +496,7 @@ __device__ __forceinline__ void zr_r2c_store(cpx_t<T> (&v)[EPTZ], cpx_t<T>* outl
                 const C zk = v[m];
                 const C zr = cconj(lds[base + ((M - k) & (M - 1))]);
                 const C sm = cadd(zk, zr), d = csub(zk, zr);
-                const C ed = cmul(twN[k], d);
+                const C ed = cmul(ek[m], d);
                 r = cmk<C>(R(0.5) * (sm.x + ed.y), R(0.5) * (sm.y - ed.x));
             }
             outline[k] = r;
@@ -513,15 +515,14 @@ __global__ void __launch_bounds__(64 * WPB, OCC) passz_rhs_kernel(PZArgs a) {
     constexpr int SY = SYNC_WAVE;
     constexpr int MP = zlds<M, EPTZ>();
     __shared__ C lds[WPB * LPW * MP];
-    __shared__ C twm_s[M];
-    __shared__ C twn_s[M];
-    for (int i = threadIdx.x; i < M; i += blockDim.x) {
-        twm_s[i] = ((const C*)a.twM)[i];
-        twn_s[i] = ((const C*)a.twN)[i];
-    }
-    __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int sub = lane / TPF, t = lane % TPF;
+    // per-thread twiddles, fixed for every line this thread handles
+    C wz[zntw<M, EPTZ>()];
+    fft_load_base_twiddles<M, EPTZ>(wz, t, (const C*)a.twM);
+    C ek[EPTZ];  // e^{-2 pi i k / N}, k = t + TPF m
+#pragma unroll
+    for (int m = 0; m < EPTZ; ++m) ek[m] = ((const C*)a.twN)[t + TPF * m];
     const int base = (wv * LPW + sub) * MP;
     const int kmax = a.kmax, NZP = a.NZP;
     const long long nlines = (long long)N * N;
@@ -549,7 +550,7 @@ __global__ void __launch_bounds__(64 * WPB, OCC) passz_rhs_kernel(PZArgs a) {
             C nxt[EPTZ];
             if (f < 5) zr_load<T, N, EPTZ>(nxt, in + (f + 1) * fstride, kmax, t);
             C v[EPTZ];
-            zr_c2r<T, N, EPTZ, SY>(v, cur, t, lds, base, twm_s, twn_s);
+            zr_c2r<T, N, EPTZ, SY>(v, cur, t, lds, base, wz, ek);
 #pragma unroll
             for (int m = 0; m < EPTZ; ++m) {
                 const C w = v[m];
@@ -588,7 +589,7 @@ __global__ void __launch_bounds__(64 * WPB, OCC) passz_rhs_kernel(PZArgs a) {
         }
         C* out = out0 + lid * NZP;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) zr_r2c_store<T, N, EPTZ, SY>(pr[c], out + c * fstride, kmax, NZP, t, lds, base, twm_s, twn_s, live);
+        for (int c = 0; c < 3; ++c) zr_r2c_store<T, N, EPTZ, SY>(pr[c], out + c * fstride, kmax, NZP, t, lds, base, wz, ek, live);
     }
     if (a.track_max) {
         mx *= a.scale;

@@ -93,7 +93,19 @@ def main():
     ap.add_argument("--prof-iters", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=512)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run one device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe_mib:
+        import torch
+
+        n = args.copy_probe_mib << 20
+        a = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs one box on one GPU; --gpus N>1 is not supported "
                                    "(multi-GPU slab decomposition is a design note only)"}))
