@@ -423,10 +423,10 @@ int nsg_create(const nsg_params* p, nsg_ctx** out) {
     const size_t nstate = 3 * NK * NK * NZP;
     const size_t ebulk = c->fp32 ? sizeof(float2) : sizeof(double2);
     // B1: A1 [6][N][NK][NZP], A3 [3][N][N][NZP], real AoS [N^3][3]  (FP64 sized: IC/realspace paths run FP64)
-    size_t b1 = std::max(6 * Nl * NK * NZP, 3 * Nl * Nl * NZP) * sizeof(double2);
+    size_t b1 = std::max(6 * Nl * (size_t)xplane(c->NK, c->NZP), 3 * Nl * Nl * NZP) * sizeof(double2);
     b1 = std::max(b1, 3 * Nl * Nl * Nl * sizeof(double));
     // B2: A2 [6][N][N][NZP], A4, full AoS spectral staging [N][N][N/2+1][3], real AoS
-    size_t b2 = 6 * Nl * Nl * NZP * sizeof(double2);
+    size_t b2 = std::max(6 * Nl * Nl * NZP, 3 * Nl * (size_t)xplane(c->NK, c->NZP)) * sizeof(double2);
     b2 = std::max(b2, 3 * Nl * Nl * (Nl / 2 + 1) * sizeof(double2));
     b2 = std::max(b2, 3 * Nl * Nl * Nl * sizeof(double));
     auto alloc = [&](void** ptr, size_t bytes) -> int {

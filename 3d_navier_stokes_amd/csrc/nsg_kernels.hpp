@@ -36,13 +36,15 @@ struct DevScalars {
 //   A1 / A4     : [F][N][NK][NZP]             -- x-major; measured faster than x-row tiles of
 //                 4..32 rows or the y-major order, which slow the y passes more than they help x
 // ------------------------------------------------------------------------------------------
-template <int N> constexpr int xtile() { return 1; }
+// A1/A4 x-plane stride = NK*NZP + XPAD elements (XPAD breaks the alignment of x rows written by
+// one store instruction)
+constexpr int XPAD = 0;
+__host__ __device__ __forceinline__ long long xplane(int NK, int NZP) { return (long long)NK * NZP + XPAD; }
 __device__ __forceinline__ long long sidx(int ix, int iy, int kz, int NK, int NZP) {
     return ((long long)iy * NK + ix) * NZP + kz;
 }
 template <int N> __device__ __forceinline__ long long xidx(int x, int iy, int kz, int NK, int NZP) {
-    constexpr int XB = xtile<N>();
-    return (((long long)(x / XB) * NK + iy) * XB + (x % XB)) * NZP + kz;
+    return (long long)x * xplane(NK, NZP) + (long long)iy * NZP + kz;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -194,7 +196,7 @@ __global__ void __launch_bounds__(N / EPT * W) pass1_kernel(P1Args a) {
     const int f1 = (a.sel == 1) ? 3 : 6;
     C* out = (C*)a.out;
     const C* tw = (const C*)a.twN;
-    const long long oplane = (long long)N * NK * NZP;
+    const long long oplane = (long long)N * xplane(NK, NZP);
     for (int f = f0; f < f1; ++f) {
         C v[EPT];
         using R = T;
@@ -244,10 +246,11 @@ __global__ void __launch_bounds__(N / EPT * W) passy_kernel(PYArgs a) {
     const int kz = blockIdx.x * W + w;
     const int x = blockIdx.y, f = blockIdx.z;
     const int kmax = a.kmax, NK = a.NK, NZP = a.NZP;
-    const int rin = (DIR > 0) ? NK : N, rout = (DIR > 0) ? N : NK;
-    // DIR>0: in = A1 (x-tiled), out = A2 plain;  DIR<0: in = A3 plain, out = A4 (x-tiled)
-    const C* in = (const C*)a.in + (long long)f * N * rin * NZP;
-    C* out = (C*)a.out + (long long)f * N * rout * NZP;
+    // DIR>0: in = A1, out = A2;  DIR<0: in = A3, out = A4 (A1/A4 planes padded by XPAD)
+    const long long fin = (DIR > 0) ? (long long)N * xplane(NK, NZP) : (long long)N * N * NZP;
+    const long long fout = (DIR > 0) ? (long long)N * N * NZP : (long long)N * xplane(NK, NZP);
+    const C* in = (const C*)a.in + (long long)f * fin;
+    C* out = (C*)a.out + (long long)f * fout;
     C v[EPT];
 #pragma unroll
     for (int m = 0; m < EPT; ++m) {
@@ -635,7 +638,7 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
     const int kmax = a.kmax, NK = a.NK, NZP = a.NZP;
     const int ky = k_of_cidx(iy, kmax, NK);
     const long long plane = (long long)NK * NK * NZP;
-    const long long iplane = (long long)N * NK * NZP;
+    const long long iplane = (long long)N * xplane(NK, NZP);
     const C* in = (const C*)a.in;
     const C* tw = (const C*)a.twN;
 

@@ -43,6 +43,25 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+KERNEL_OF_PASS = [r"pass1_kernel<{T}, {N},", r"passy_kernel<{T}, {N}, .*, 1>$", r"passz_rhs_kernel<{T}, {N},",
+                  r"passy_kernel<{T}, {N}, .*, -1>$", r"pass5_kernel<{T}, {N},"]
+
+
+def measured_traffic(N, precision, pass_index):
+    """HBM bytes per launch of a pass kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary (calibrated against a plain device copy), or None when no summary covers it."""
+    import re
+
+    path = os.path.join(ROOT, "profiles", f"step{N}_hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    pat = KERNEL_OF_PASS[pass_index].format(T="double" if precision == "fp64" else "float", N=N)
+    for name, v in json.load(open(path))["kernels"].items():
+        if re.match(pat, name):
+            return v["traffic_bytes"]
+    return None
+
+
 def run_mode(nsg, N, precision, steps, warmup, prof_iters):
     s = nsg.Solver(N, nu=1e-3, dt=1e-3, scheme="rk4cn", precision=precision, cfl=0.5, force_eps=0.1,
                    force_kf=2.5)
@@ -61,7 +80,8 @@ def run_mode(nsg, N, precision, steps, warmup, prof_iters):
     dom = max(range(5), key=lambda i: ms[i])
     achieved = by[dom] / (ms[dom] * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": PASS_NAMES[dom], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(N, precision, dom),
             "bytes_per_launch": by[dom], "ms_per_launch": ms[dom]}
     step_gb = s.step_bytes() / 1e9
     out = {"steps_per_s": steps / el, "ms_per_step": 1e3 * el / steps, "roofline": roof, "passes": passes,
