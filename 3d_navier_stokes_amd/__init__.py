@@ -84,6 +84,31 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.nsg_profile_passes.argtypes = [vp, ctypes.c_int, dp, dp]
     lib.nsg_step_bytes.argtypes = [vp]
     lib.nsg_step_bytes.restype = ctypes.c_double
+    lib.nsg_schedule_count.argtypes = [ctypes.c_int]
+    lib.nsg_set_schedule.argtypes = [vp, ctypes.c_int]
+    lib.nsg_autotune.argtypes = [vp, ctypes.c_int, ctypes.c_int, P(ctypes.c_int), dp]
+    # ensembles
+    lib.nsg_ensemble_create.argtypes = [P(_Params), ctypes.c_int, P(vp)]
+    lib.nsg_ensemble_destroy.argtypes = [vp]
+    lib.nsg_ensemble_last_error.argtypes = [vp]
+    lib.nsg_ensemble_last_error.restype = ctypes.c_char_p
+    lib.nsg_ensemble_group_count.argtypes = [vp]
+    lib.nsg_ensemble_group_of.argtypes = [vp, ctypes.c_int, P(ctypes.c_int), P(ctypes.c_int)]
+    lib.nsg_ensemble_n_spect.argtypes = [vp, ctypes.c_int]
+    lib.nsg_ensemble_init_state.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_uint64]
+    for name in ("nsg_ensemble_set_state", "nsg_ensemble_get_state"):
+        getattr(lib, name).argtypes = [vp, ctypes.c_int, vp]
+    lib.nsg_ensemble_measures.argtypes = [vp, ctypes.c_int, P(_Totals), vp, vp, ctypes.c_int, ctypes.c_int]
+    lib.nsg_ensemble_realspace.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
+    lib.nsg_ensemble_get_time.argtypes = [vp, ctypes.c_int, dp, dp, dp]
+    lib.nsg_ensemble_step.argtypes = [vp, ctypes.c_long]
+    lib.nsg_ensemble_step_serial.argtypes = [vp, ctypes.c_long]
+    lib.nsg_ensemble_time_steps.argtypes = [vp, ctypes.c_long, ctypes.c_int, dp]
+    lib.nsg_ensemble_autotune.argtypes = [vp, ctypes.c_int, ctypes.c_int, P(ctypes.c_int)]
+    lib.nsg_ensemble_set_schedule.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    lib.nsg_ensemble_step_bytes.argtypes = [vp]
+    lib.nsg_ensemble_step_bytes.restype = ctypes.c_double
     _lib = lib
     return lib
 
@@ -100,6 +125,26 @@ def _ptr(a: np.ndarray) -> ctypes.c_void_p:
     return ctypes.c_void_p(a.ctypes.data)
 
 
+def make_params(N: int, nu: float = 1.0, dt: float = 1e-3, visc_pow: int = 1, scheme: str = "rk4cn",
+                precision: str = "fp64", cfl: float = 0.0, force_eps: float = 0.0, force_kf: float = 2.5,
+                t0: float = 0.0, max_speed: float = 0.0, device: int = 0) -> _Params:
+    """An `nsg_params` struct from the library defaults overridden by the keyword values."""
+    lib = load_library()
+    p = _Params()
+    lib.nsg_default_params(ctypes.byref(p))
+    p.N[0] = p.N[1] = p.N[2] = int(N)
+    p.nu, p.visc_pow, p.dt, p.t0 = float(nu), int(visc_pow), float(dt), float(t0)
+    p.scheme, p.precision = SCHEMES[scheme], PRECISIONS[precision]
+    p.cfl, p.force_eps, p.force_kf = float(cfl), float(force_eps), float(force_kf)
+    p.max_speed, p.device = float(max_speed), int(device)
+    return p
+
+
+def schedule_count(N: int) -> int:
+    """Number of FFT launch schedules the library has for size N (0 if N is unsupported)."""
+    return load_library().nsg_schedule_count(int(N))
+
+
 class Solver:
     """One periodic box resident on one GPU (a `nsg_ctx`)."""
 
@@ -107,13 +152,8 @@ class Solver:
                  precision: str = "fp64", cfl: float = 0.0, force_eps: float = 0.0, force_kf: float = 2.5,
                  t0: float = 0.0, max_speed: float = 0.0, device: int = 0):
         self.lib = load_library()
-        p = _Params()
-        self.lib.nsg_default_params(ctypes.byref(p))
-        p.N[0] = p.N[1] = p.N[2] = int(N)
-        p.nu, p.visc_pow, p.dt, p.t0 = float(nu), int(visc_pow), float(dt), float(t0)
-        p.scheme, p.precision = SCHEMES[scheme], PRECISIONS[precision]
-        p.cfl, p.force_eps, p.force_kf = float(cfl), float(force_eps), float(force_kf)
-        p.max_speed, p.device = float(max_speed), int(device)
+        p = make_params(N, nu=nu, dt=dt, visc_pow=visc_pow, scheme=scheme, precision=precision, cfl=cfl,
+                        force_eps=force_eps, force_kf=force_kf, t0=t0, max_speed=max_speed, device=device)
         self.N = int(N)
         self.params = p
         h = ctypes.c_void_p()
@@ -214,3 +254,135 @@ class Solver:
 
     def step_bytes(self) -> float:
         return self.lib.nsg_step_bytes(self._h)
+
+    def set_schedule(self, sched: int):
+        self._check(self.lib.nsg_set_schedule(self._h, int(sched)))
+
+    def autotune(self, iters: int = 3, use_cache: bool = True):
+        """Time every FFT schedule of this size on the current state (restored afterwards) and keep
+        the fastest; returns (chosen schedule, ms per step of each schedule or None if cached)."""
+        n = schedule_count(self.N)
+        ch = ctypes.c_int()
+        ms = (ctypes.c_double * max(n, 1))(*([-1.0] * max(n, 1)))
+        self._check(self.lib.nsg_autotune(self._h, int(iters), int(use_cache), ctypes.byref(ch), ms))
+        times = list(ms)[:n]
+        return ch.value, (None if all(t < 0 for t in times) else times)
+
+
+class Ensemble:
+    """Many independent boxes on one GPU (a `nsg_ensemble`).  `boxes` is a list of dicts of
+    `make_params` keywords, one per box; boxes sharing N, precision, scheme, visc_pow and forcing
+    are stepped as one batch, and one step of all batches is a single HIP graph replay."""
+
+    def __init__(self, boxes):
+        self.lib = load_library()
+        n = len(boxes)
+        if n < 1:
+            raise ValueError("an ensemble needs at least one box")
+        arr = (_Params * n)(*[make_params(**b) for b in boxes])
+        self.N = [int(b["N"]) for b in boxes]
+        h = ctypes.c_void_p()
+        rc = self.lib.nsg_ensemble_create(arr, n, ctypes.byref(h))
+        self._h = h
+        if rc != 0:
+            msg = self.lib.nsg_ensemble_last_error(h).decode() if h.value else "nsg_ensemble_create failed"
+            if h.value:
+                self.lib.nsg_ensemble_destroy(h)
+                self._h = ctypes.c_void_p()
+            raise NSGError(rc, msg)
+        self.nbox = n
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise NSGError(rc, self.lib.nsg_ensemble_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self.lib.nsg_ensemble_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __len__(self):
+        return self.nbox
+
+    @property
+    def groups(self) -> int:
+        return self.lib.nsg_ensemble_group_count(self._h)
+
+    def group_of(self, box: int):
+        g, i = ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.nsg_ensemble_group_of(self._h, int(box), ctypes.byref(g), ctypes.byref(i)))
+        return g.value, i.value
+
+    def spectral_shape(self, box: int):
+        N = self.N[box]
+        return (N, N, N // 2 + 1, 3)
+
+    def init(self, box: int, kind: str, k0: float = 4.0, energy: float = 0.5, seed: int = 1):
+        self._check(self.lib.nsg_ensemble_init_state(self._h, int(box), ICS[kind], float(k0), float(energy), int(seed)))
+
+    def set_state(self, box: int, u_hat: np.ndarray):
+        a = np.ascontiguousarray(u_hat, dtype=np.complex128)
+        if a.shape != self.spectral_shape(box):
+            raise ValueError(f"u_hat shape {a.shape} != {self.spectral_shape(box)}")
+        self._check(self.lib.nsg_ensemble_set_state(self._h, int(box), _ptr(a)))
+
+    def get_state(self, box: int) -> np.ndarray:
+        out = np.empty(self.spectral_shape(box), dtype=np.complex128)
+        self._check(self.lib.nsg_ensemble_get_state(self._h, int(box), _ptr(out)))
+        return out
+
+    def measures(self, box: int, quirk: bool = False):
+        ns = self.lib.nsg_ensemble_n_spect(self._h, int(box))
+        t = _Totals()
+        ek = np.zeros(ns)
+        ens = np.zeros(ns)
+        self._check(self.lib.nsg_ensemble_measures(self._h, int(box), ctypes.byref(t), _ptr(ek), _ptr(ens), ns,
+                                                   int(quirk)))
+        return Totals(t.energy, t.enstrophy, t.palinstrophy, t.helicity, t.dissipation), ek, ens
+
+    def realspace(self, box: int, which: int = 0) -> np.ndarray:
+        N = self.N[box]
+        out = np.empty((N, N, N, 3), dtype=np.float64)
+        self._check(self.lib.nsg_ensemble_realspace(self._h, int(box), int(which), _ptr(out)))
+        return out
+
+    def time(self, box: int):
+        t, dt, mx = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.nsg_ensemble_get_time(self._h, int(box), ctypes.byref(t), ctypes.byref(dt),
+                                                   ctypes.byref(mx)))
+        return t.value, dt.value, mx.value
+
+    def step(self, nsteps: int = 1):
+        self._check(self.lib.nsg_ensemble_step(self._h, int(nsteps)))
+
+    def step_serial(self, nsteps: int = 1):
+        self._check(self.lib.nsg_ensemble_step_serial(self._h, int(nsteps)))
+
+    def time_steps(self, nsteps: int, serial: bool = False) -> float:
+        """Milliseconds for nsteps steps of every box (HIP events), graph or per-box launches."""
+        ms = ctypes.c_double()
+        self._check(self.lib.nsg_ensemble_time_steps(self._h, int(nsteps), int(serial), ctypes.byref(ms)))
+        return ms.value
+
+    def autotune(self, iters: int = 3, use_cache: bool = True):
+        ch = (ctypes.c_int * max(self.groups, 1))()
+        self._check(self.lib.nsg_ensemble_autotune(self._h, int(iters), int(use_cache), ch))
+        return list(ch)[:self.groups]
+
+    def set_schedule(self, group: int, sched: int):
+        self._check(self.lib.nsg_ensemble_set_schedule(self._h, int(group), int(sched)))
+
+    def step_bytes(self) -> float:
+        return self.lib.nsg_ensemble_step_bytes(self._h)

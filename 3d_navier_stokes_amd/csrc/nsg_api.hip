@@ -1,17 +1,23 @@
-// nsg_api.hip -- C ABI implementation (include/nsg.h): context, launch sequencing, transfers.
+// nsg_api.hip -- C ABI implementation (include/nsg.h): batches of boxes, launch sequencing,
+// transfers, the ensemble (several batches stepped by one HIP graph) and the schedule autotuner.
 //
-// One RK4 step = 4 stages x 5 passes (see nsg_kernels.hpp) + three one-thread bookkeeping
-// kernels, all on the context's stream; nothing is copied to the host inside nsg_step except the
-// final status scalars.  The reference counterpart is SpectralSolve's loop body
+// A context holds a batch of nbox periodic boxes of one size N and precision (nsg_create makes a
+// batch of one).  One RK4 step of a batch = 4 stages x 5 passes (see nsg_kernels.hpp), each pass
+// one launch over every box of the batch, plus one-thread bookkeeping kernels; nothing is copied
+// to the host inside a step.  The reference counterpart is SpectralSolve's loop body
 // (Solver/src/solver.c:125-194) calling RK4Step (:505) -> NonlinearRHSBatch (:620).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <type_traits>
 #include <vector>
 
@@ -21,34 +27,48 @@
 using namespace nsg;
 
 // ------------------------------------------------------------------------------------------
-// per-N launch configuration
-//   EPT   : elements per thread of the length-N column transforms (x and y passes)
-//   EPTZ64/EPTZ32 : elements per thread of the length-N/2 line transform (z pass)
-//   W64/W32 : kz columns per block in the x/y passes (128-byte row segments)
+// FFT schedules.  EX/EY/EZ: elements per thread of the x, y (length N) and z (length N/2)
+// transforms, which fixes the radix sequence (nsg_fft.hpp); WX/WY: kz columns per block of the x
+// and y passes.  Schedule 0 is the default; the ensemble sizes 32..96 have two alternatives that
+// nsg_autotune times: 1 = shorter radices (more threads per transform), 2 = wider column tiles.
 // ------------------------------------------------------------------------------------------
-template <int N> struct Cfg;
-// EX/EY/EZ: elements per thread of the x, y (length N) and z (length N/2) transforms;
-// WX/WY: kz columns per block of the x and y passes (FP64 / FP32)
-#define NSG_CFG(n, ex64, ex32, ey64, ey32, ez64, ez32)                       \
-    template <> struct Cfg<n> {                                              \
-        static constexpr int EX64 = ex64, EX32 = ex32, EY64 = ey64, EY32 = ey32; \
-        static constexpr int EZ64 = ez64, EZ32 = ez32;                       \
-        static constexpr int WX64 = 8, WX32 = 16, WY64 = 8, WY32 = 16;       \
-    };
-//      N     EX64 EX32 EY64 EY32 EZ64 EZ32
-NSG_CFG(8,    8,   8,   8,   8,   4,   4)
-NSG_CFG(12,   12,  12,  12,  12,  6,   6)
-NSG_CFG(16,   8,   8,   8,   8,   4,   8)
-NSG_CFG(24,   12,  12,  12,  12,  12,  12)
-NSG_CFG(32,   8,   8,   8,   8,   8,   8)
-NSG_CFG(48,   12,  12,  12,  12,  12,  12)
-NSG_CFG(64,   8,   8,   8,   8,   8,   8)
-NSG_CFG(96,   12,  12,  12,  12,  12,  12)
-NSG_CFG(128,  8,   8,   8,   8,   8,   8)
-NSG_CFG(256,  8,   8,   8,   8,   4,   8)
-NSG_CFG(512,  8,   8,   8,   16,  4,   8)
-NSG_CFG(1024, 16,  16,  16,  16,  8,   8)
-#undef NSG_CFG
+struct SchedV {
+    int ex64, ex32, ey64, ey32, ez64, ez32, wx64, wx32, wy64, wy32;
+};
+constexpr int nzp_of(int N) { return ((N / 3 + 1 + 15) / 16) * 16; }
+constexpr SchedV sched0(int N) {
+    //                     ex64 ex32 ey64 ey32 ez64 ez32 wx64 wx32 wy64 wy32
+    return N == 8     ? SchedV{8, 8, 8, 8, 4, 4, 8, 16, 8, 16}
+           : N == 12  ? SchedV{12, 12, 12, 12, 6, 6, 8, 16, 8, 16}
+           : N == 16  ? SchedV{8, 8, 8, 8, 4, 8, 8, 16, 8, 16}
+           : N == 24  ? SchedV{12, 12, 12, 12, 12, 12, 8, 16, 8, 16}
+           : N == 32  ? SchedV{8, 8, 8, 8, 8, 8, 8, 16, 8, 16}
+           : N == 48  ? SchedV{12, 12, 12, 12, 12, 12, 8, 16, 8, 16}
+           : N == 64  ? SchedV{8, 8, 8, 8, 8, 8, 8, 16, 8, 16}
+           : N == 96  ? SchedV{12, 12, 12, 12, 12, 12, 8, 16, 8, 16}
+           : N == 128 ? SchedV{8, 8, 8, 8, 8, 8, 8, 16, 8, 16}
+           : N == 256 ? SchedV{8, 8, 8, 8, 4, 8, 8, 16, 8, 16}
+           : N == 512 ? SchedV{8, 8, 8, 16, 4, 8, 8, 16, 8, 16}
+                      : SchedV{16, 16, 16, 16, 8, 8, 8, 16, 8, 16};
+}
+constexpr int nsched(int N) { return (N == 32 || N == 48 || N == 64 || N == 96) ? 3 : 1; }
+constexpr SchedV sched_of(int N, int S) {
+    SchedV v = sched0(N);
+    if (S == 1) {
+        const int e = (N % 3 == 0) ? 6 : 4;
+        v.ex64 = v.ex32 = v.ey64 = v.ey32 = v.ez64 = v.ez32 = e;
+    } else if (S == 2) {
+        v.wx64 = v.wy64 = 16;
+        v.wx32 = v.wy32 = (nzp_of(N) % 32 == 0) ? 32 : 16;
+    }
+    return v;
+}
+template <typename T> static constexpr bool is64() { return std::is_same<T, double>::value; }
+template <typename T, int N, int S> static constexpr int EXof() { return is64<T>() ? sched_of(N, S).ex64 : sched_of(N, S).ex32; }
+template <typename T, int N, int S> static constexpr int EYof() { return is64<T>() ? sched_of(N, S).ey64 : sched_of(N, S).ey32; }
+template <typename T, int N, int S> static constexpr int EZof() { return is64<T>() ? sched_of(N, S).ez64 : sched_of(N, S).ez32; }
+template <typename T, int N, int S> static constexpr int WXof() { return is64<T>() ? sched_of(N, S).wx64 : sched_of(N, S).wx32; }
+template <typename T, int N, int S> static constexpr int WYof() { return is64<T>() ? sched_of(N, S).wy64 : sched_of(N, S).wy32; }
 
 template <typename F> static int dispatch_N(int N, F&& f) {
     switch (N) {
@@ -67,25 +87,40 @@ template <typename F> static int dispatch_N(int N, F&& f) {
         default: return NSG_ERR_INVALID;
     }
 }
+// f(N as a constant, schedule as a constant)
+template <typename F> static int dispatch_NS(int N, int S, F&& f) {
+    return dispatch_N(N, [&](auto n) {
+        constexpr int NN = decltype(n)::value;
+        if constexpr (nsched(NN) > 1) {
+            if (S == 1) return f(n, std::integral_constant<int, 1>{});
+            if (S == 2) return f(n, std::integral_constant<int, 2>{});
+        }
+        return f(n, std::integral_constant<int, 0>{});
+    });
+}
 
 struct nsg_ctx {
-    nsg_params p;
-    int N = 0, kmax = 0, NK = 0, NZP = 0, nspect = 0;
+    nsg_params p;  // parameters of box 0; the ones a batch shares apply to every box
+    int N = 0, kmax = 0, NK = 0, NZP = 0, nspect = 0, nbox = 1;
+    int sched = 0;
+    bool fp32 = false, any_cfl = false;
     int ncu = 256, zblocks_per_cu = 8;  // persistent z-pass grid = ncu * zblocks_per_cu blocks
     int zocc = 1;                       // >= 3: z kernel compiled for 3 waves/SIMD (fewer VGPRs)
-    bool fp32 = false;
     hipStream_t s = nullptr;
-    double2* uh = nullptr;   // state [3][NK][NK][NZP]
-    double2* aux = nullptr;  // second FP64 state (nsg_rhs input)
+    // per-box element strides of the state/K arrays and of A1..A4
+    long long sbox = 0, a1box = 0, a2box = 0, a3box = 0, a4box = 0;
+    double2* uh = nullptr;   // states [nbox][3][NK][NK][NZP]
+    double2* aux = nullptr;  // a second FP64 state per box (nsg_rhs input, autotuner backup)
     void* K[3] = {nullptr, nullptr, nullptr};
-    void* B1 = nullptr;
-    void* B2 = nullptr;
+    void* B1 = nullptr;  // A1 / A3 of every box, or one box's real AoS field
+    void* B2 = nullptr;  // A2 / A4 of every box, or one box's AoS staging
     double2 *twN64 = nullptr, *twM64 = nullptr;
     float2 *twN32 = nullptr, *twM32 = nullptr;
-    DevScalars* sc = nullptr;
-    double* fslots = nullptr;
+    DevScalars* sc = nullptr;  // [nbox]
+    double* fslots = nullptr;  // [nbox][NK]
     double* part = nullptr;
     size_t part_n = 0;
+    std::vector<double> max_speed;  // per-box blow-up limit (0 = none)
     std::string err;
     // profiling
     bool prof = false;
@@ -175,38 +210,37 @@ static inline void prof_end(nsg_ctx* c, int k) {
 }
 
 // ------------------------------------------------------------------------------------------
-// pass launchers
+// pass launchers over boxes [b0, b0 + nb) of a batch
 // ------------------------------------------------------------------------------------------
 template <typename T> static const void* twN_of(nsg_ctx* c) {
-    if constexpr (std::is_same<T, double>::value) return c->twN64; else return c->twN32;
+    if constexpr (is64<T>()) return c->twN64; else return c->twN32;
 }
 template <typename T> static const void* twM_of(nsg_ctx* c) {
-    if constexpr (std::is_same<T, double>::value) return c->twM64; else return c->twM32;
+    if constexpr (is64<T>()) return c->twM64; else return c->twM32;
 }
-template <typename T> static constexpr bool is64() { return std::is_same<T, double>::value; }
-template <typename T, int N> static constexpr int WXof() { return is64<T>() ? Cfg<N>::WX64 : Cfg<N>::WX32; }
-template <typename T, int N> static constexpr int WYof() { return is64<T>() ? Cfg<N>::WY64 : Cfg<N>::WY32; }
-template <typename T, int N> static constexpr int EXof() { return is64<T>() ? Cfg<N>::EX64 : Cfg<N>::EX32; }
-template <typename T, int N> static constexpr int EYof() { return is64<T>() ? Cfg<N>::EY64 : Cfg<N>::EY32; }
-template <typename T, int N> static constexpr int EZof() { return is64<T>() ? Cfg<N>::EZ64 : Cfg<N>::EZ32; }
+template <typename T> static constexpr size_t esz() { return sizeof(cpx_t<T>); }
 
-// grid of the x passes: 2-D (tiles, rows), or 1-D XCD-paired when a tile row is under 128 bytes
-template <typename T, int W> static dim3 xgrid(int ntiles, int nrows) {
-    if (W * sizeof(cpx_t<T>) >= 128) return dim3(ntiles, nrows);
+// grid of the x passes: (tiles, rows, boxes), or XCD-paired tiles when a tile row is under 128 B
+template <typename T, int W> static dim3 xgrid(int ntiles, int nrows, int nb) {
+    if (W * sizeof(cpx_t<T>) >= 128) return dim3(ntiles, nrows, nb);
     const long long pairs = (long long)(ntiles / 2) * nrows;
-    return dim3((unsigned)(16 * ((pairs + 7) / 8)));
+    return dim3((unsigned)(16 * ((pairs + 7) / 8)), 1, nb);
 }
 
-template <typename T, int N>
-static int launch_pass1(nsg_ctx* c, const double2* uh, const void* kprev, int stage, int sel, int force) {
-    constexpr int EPT = EXof<T, N>(), W = WXof<T, N>(), TPF = N / EPT;
+template <typename T, int N, int S>
+static int launch_pass1(nsg_ctx* c, const double2* uh, int b0, int nb, const void* kprev, int stage, int sel, int force) {
+    constexpr int EPT = EXof<T, N, S>(), W = WXof<T, N, S>(), TPF = N / EPT;
     static_assert(TPF * W <= 1024, "block too large");
     P1Args a;
-    a.uh = uh; a.kprev = kprev; a.out = c->B1; a.twN = twN_of<T>(c); a.sc = c->sc; a.fslots = c->fslots;
-    a.nu = c->p.nu; a.kf2 = c->p.force_kf * c->p.force_kf;
+    a.uh = uh + b0 * c->sbox;
+    a.kprev = kprev ? (const char*)kprev + b0 * c->sbox * esz<T>() : nullptr;
+    a.out = (char*)c->B1 + b0 * c->a1box * esz<T>();
+    a.twN = twN_of<T>(c); a.sc = c->sc + b0; a.fslots = c->fslots + (long long)b0 * c->NK;
+    a.kf2 = c->p.force_kf * c->p.force_kf;
+    a.sbox = c->sbox; a.obox = c->a1box;
     a.visc_pow = c->p.visc_pow; a.scheme = c->p.scheme; a.stage = stage; a.force = force; a.sel = sel;
     a.kmax = c->kmax; a.NK = c->NK; a.NZP = c->NZP;
-    const dim3 grid = xgrid<T, W>(c->NZP / W, c->NK);
+    const dim3 grid = xgrid<T, W>(c->NZP / W, c->NK, nb);
     prof_begin(c);
     hipLaunchKernelGGL((pass1_kernel<T, N, EPT, W>), grid, dim3(TPF * W), 0, c->s, a);
     LAUNCHCHK(c);
@@ -214,33 +248,42 @@ static int launch_pass1(nsg_ctx* c, const double2* uh, const void* kprev, int st
     return NSG_OK;
 }
 
-template <typename T, int N, int DIR>
-static int launch_passy(nsg_ctx* c, const void* in, void* out, int F) {
-    constexpr int EPT = EYof<T, N>(), W = WYof<T, N>(), TPF = N / EPT;
+// DIR > 0: A1 (B1) -> A2 (B2) on F fields;  DIR < 0: A3 (B1) -> A4 (B2) on 3 fields
+template <typename T, int N, int S, int DIR>
+static int launch_passy(nsg_ctx* c, int b0, int nb, int F) {
+    constexpr int EPT = EYof<T, N, S>(), W = WYof<T, N, S>(), TPF = N / EPT;
     static_assert(TPF * W <= 1024, "block too large");
     PYArgs a;
-    a.in = in; a.out = out; a.twN = twN_of<T>(c); a.kmax = c->kmax; a.NK = c->NK; a.NZP = c->NZP;
+    a.ibox = DIR > 0 ? c->a1box : c->a3box;
+    a.obox = DIR > 0 ? c->a2box : c->a4box;
+    a.in = (const char*)c->B1 + b0 * a.ibox * esz<T>();
+    a.out = (char*)c->B2 + b0 * a.obox * esz<T>();
+    a.F = F;
+    a.twN = twN_of<T>(c); a.kmax = c->kmax; a.NK = c->NK; a.NZP = c->NZP;
     prof_begin(c);
-    hipLaunchKernelGGL((passy_kernel<T, N, EPT, W, DIR>), dim3(c->NZP / W, N, F), dim3(TPF * W), 0, c->s, a);
+    hipLaunchKernelGGL((passy_kernel<T, N, EPT, W, DIR>), dim3(c->NZP / W, N, F * nb), dim3(TPF * W), 0, c->s, a);
     LAUNCHCHK(c);
     prof_end(c, DIR > 0 ? 1 : 3);
     return NSG_OK;
 }
 
-// RHS z pass: persistent wavefronts (WPB per block) when a line fits in a wavefront
-template <typename T, int N> static constexpr bool zrhs_persistent() {
-    constexpr int TPFZ = N / 2 / EZof<T, N>();
+// RHS z pass: persistent wavefronts when a line fits in a wavefront and N/2 is a power of two
+template <typename T, int N, int S> static constexpr bool zrhs_persistent() {
+    constexpr int TPFZ = N / 2 / EZof<T, N, S>();
     return TPFZ <= 64 && 64 % TPFZ == 0 && ((N / 2) & (N / 2 - 1)) == 0;
 }
 
-template <typename T, int N>
-static int launch_passz_rhs(nsg_ctx* c, const void* in, void* out, double scale, int track_max) {
-    constexpr int EPTZ = EZof<T, N>(), TPFZ = N / 2 / EPTZ, LPW = 64 / TPFZ, WPB = 4;
+template <typename T, int N, int S>
+static int launch_passz_rhs(nsg_ctx* c, int b0, int nb, double scale, int track_max) {
+    constexpr int EPTZ = EZof<T, N, S>(), TPFZ = N / 2 / EPTZ, LPW = 64 / TPFZ, WPB = 4;
     static_assert(EPTZ >= 1 && LPW >= 1, "bad z configuration");
     PZArgs a;
-    a.in = in; a.out = out; a.twM = twM_of<T>(c); a.twN = twN_of<T>(c); a.sc = c->sc;
+    a.in = (const char*)c->B2 + b0 * c->a2box * esz<T>();
+    a.out = (char*)c->B1 + b0 * c->a3box * esz<T>();
+    a.ibox = c->a2box; a.obox = c->a3box; a.nbox = nb;
+    a.twM = twM_of<T>(c); a.twN = twN_of<T>(c); a.sc = c->sc + b0;
     a.scale = scale; a.kmax = c->kmax; a.NZP = c->NZP; a.track_max = track_max;
-    const long long groups = ((long long)N * N + LPW - 1) / LPW;
+    const long long groups = ((long long)N * N * nb + LPW - 1) / LPW;
     long long blocks = (groups + WPB - 1) / WPB;
     const long long cap = (long long)c->ncu * c->zblocks_per_cu;
     if (blocks > cap) blocks = cap;
@@ -254,14 +297,18 @@ static int launch_passz_rhs(nsg_ctx* c, const void* in, void* out, double scale,
     return NSG_OK;
 }
 
-template <typename T, int N, int MODE>
-static int launch_passz(nsg_ctx* c, const void* in, void* out, double scale, int track_max) {
-    constexpr int EPTZ = EZof<T, N>(), TPFZ = N / 2 / EPTZ;
+// block-synchronous z pass (every mode); in/out point at the first box, ibox/obox are per-box
+// strides in elements of the input/output type
+template <typename T, int N, int S, int MODE>
+static int launch_passz(nsg_ctx* c, const void* in, void* out, long long ibox, long long obox, int nb, DevScalars* sc,
+                        double scale, int track_max) {
+    constexpr int EPTZ = EZof<T, N, S>(), TPFZ = N / 2 / EPTZ;
     constexpr int LPB = (256 / TPFZ) > 0 ? (256 / TPFZ) : 1;
     PZArgs a;
-    a.in = in; a.out = out; a.twM = twM_of<T>(c); a.twN = twN_of<T>(c); a.sc = c->sc;
+    a.in = in; a.out = out; a.ibox = ibox; a.obox = obox; a.nbox = nb;
+    a.twM = twM_of<T>(c); a.twN = twN_of<T>(c); a.sc = sc;
     a.scale = scale; a.kmax = c->kmax; a.NZP = c->NZP; a.track_max = track_max;
-    const long long lines = (long long)N * N;
+    const long long lines = (long long)N * N * nb;
     const unsigned blocks = (unsigned)((lines + LPB - 1) / LPB);
     prof_begin(c);
     hipLaunchKernelGGL((passz_kernel<T, N, EPTZ, LPB, MODE>), dim3(blocks), dim3(TPFZ * LPB), 0, c->s, a);
@@ -270,17 +317,24 @@ static int launch_passz(nsg_ctx* c, const void* in, void* out, double scale, int
     return NSG_OK;
 }
 
-template <typename T, int N, int MODE>
-static int launch_pass5(nsg_ctx* c, double2* uh, int stage, void* kout, int force, double norm) {
-    constexpr int EPT = EXof<T, N>(), W = WXof<T, N>(), TPF = N / EPT;
+template <typename T, int N, int S, int MODE>
+static int launch_pass5(nsg_ctx* c, double2* uh, int b0, int nb, int stage, void* kout, int force, double norm) {
+    constexpr int EPT = EXof<T, N, S>(), W = WXof<T, N, S>(), TPF = N / EPT;
     static_assert(TPF * W <= 1024, "block too large");
+    constexpr size_t e = esz<T>();
     P5Args a;
-    a.in = c->B2; a.uh = uh; a.k1 = c->K[0]; a.k2 = c->K[1]; a.k3 = c->K[2]; a.kout = kout;
-    a.twN = twN_of<T>(c); a.sc = c->sc; a.fslots = c->fslots;
-    a.nu = c->p.nu; a.kf2 = c->p.force_kf * c->p.force_kf; a.eps_in = c->p.force_eps; a.norm = norm;
+    a.in = (const char*)c->B2 + b0 * c->a4box * e;
+    a.uh = uh + b0 * c->sbox;
+    a.k1 = (const char*)c->K[0] + b0 * c->sbox * e;
+    a.k2 = (const char*)c->K[1] + b0 * c->sbox * e;
+    a.k3 = (const char*)c->K[2] + b0 * c->sbox * e;
+    a.kout = kout ? (char*)kout + b0 * c->sbox * e : nullptr;
+    a.twN = twN_of<T>(c); a.sc = c->sc + b0; a.fslots = c->fslots + (long long)b0 * c->NK;
+    a.kf2 = c->p.force_kf * c->p.force_kf; a.eps_in = c->p.force_eps; a.norm = norm;
+    a.sbox = c->sbox; a.ibox = c->a4box;
     a.visc_pow = c->p.visc_pow; a.scheme = c->p.scheme; a.stage = stage; a.force = force;
     a.kmax = c->kmax; a.NK = c->NK; a.NZP = c->NZP;
-    const dim3 grid = xgrid<T, W>(c->NZP / W, c->NK);
+    const dim3 grid = xgrid<T, W>(c->NZP / W, c->NK, nb);
     prof_begin(c);
     hipLaunchKernelGGL((pass5_kernel<T, N, EPT, W, MODE>), grid, dim3(TPF * W), 0, c->s, a);
     LAUNCHCHK(c);
@@ -288,51 +342,336 @@ static int launch_pass5(nsg_ctx* c, double2* uh, int stage, void* kout, int forc
     return NSG_OK;
 }
 
-// one RK stage s (1..4) of the nonlinear term on state `uh`, writing K_s (s<4) or the update (s=4)
-template <typename T, int N>
-static int run_stage(nsg_ctx* c, double2* uh, int stage, bool track_max, bool final_update) {
+// one RK stage s (1..4) of the nonlinear term on boxes [b0, b0+nb) of `uh`, writing K_s (s<4)
+// or the update (final_update)
+template <typename T, int N, int S>
+static int run_stage(nsg_ctx* c, double2* uh, int b0, int nb, int stage, bool track_max, bool final_update) {
     const int force = c->p.force_eps > 0.0 ? 1 : 0;
     const void* kprev = stage > 1 ? c->K[stage - 2] : nullptr;
     const double N3 = (double)N * N * N;
     int rc;
-    if ((rc = launch_pass1<T, N>(c, uh, kprev, stage, 0, force))) return rc;
-    if ((rc = launch_passy<T, N, +1>(c, c->B1, c->B2, 6))) return rc;
-    if constexpr (zrhs_persistent<T, N>()) {
-        if ((rc = launch_passz_rhs<T, N>(c, c->B2, c->B1, 1.0 / N3, track_max ? 1 : 0))) return rc;
+    if ((rc = launch_pass1<T, N, S>(c, uh, b0, nb, kprev, stage, 0, force))) return rc;
+    if ((rc = launch_passy<T, N, S, +1>(c, b0, nb, 6))) return rc;
+    if constexpr (zrhs_persistent<T, N, S>()) {
+        if ((rc = launch_passz_rhs<T, N, S>(c, b0, nb, 1.0 / N3, track_max ? 1 : 0))) return rc;
     } else {
-        if ((rc = launch_passz<T, N, ZM_RHS>(c, c->B2, c->B1, 1.0 / N3, track_max ? 1 : 0))) return rc;
+        if ((rc = launch_passz<T, N, S, ZM_RHS>(c, (const char*)c->B2 + b0 * c->a2box * esz<T>(),
+                                               (char*)c->B1 + b0 * c->a3box * esz<T>(), c->a2box, c->a3box, nb,
+                                               c->sc + b0, 1.0 / N3, track_max ? 1 : 0)))
+            return rc;
     }
-    if (track_max && c->p.cfl > 0.0) {
-        hipLaunchKernelGGL(cfl_kernel, dim3(1), dim3(1), 0, c->s, c->sc);
+    if (track_max && c->any_cfl) {
+        hipLaunchKernelGGL(cfl_kernel, dim3(1), dim3(64), 0, c->s, c->sc + b0, nb);
         LAUNCHCHK(c);
     }
-    if ((rc = launch_passy<T, N, -1>(c, c->B1, c->B2, 3))) return rc;
+    if ((rc = launch_passy<T, N, S, -1>(c, b0, nb, 3))) return rc;
     const int st = final_update ? 4 : (stage < 4 ? stage : 1);
     void* kout = (st < 4) ? c->K[st - 1] : nullptr;
-    return launch_pass5<T, N, EP_STAGE>(c, uh, st, kout, force, 1.0 / (N3 * N3));
+    return launch_pass5<T, N, S, EP_STAGE>(c, uh, b0, nb, st, kout, force, 1.0 / (N3 * N3));
 }
 
-template <typename T, int N> static int run_step(nsg_ctx* c) {
+template <typename T, int N, int S> static int run_step(nsg_ctx* c, int b0, int nb) {
     int rc;
-    hipLaunchKernelGGL(step_begin_kernel, dim3(1), dim3(1), 0, c->s, c->sc);
+    hipLaunchKernelGGL(step_begin_kernel, dim3(1), dim3(64), 0, c->s, c->sc + b0, nb);
     LAUNCHCHK(c);
     for (int s = 1; s <= 4; ++s)
-        if ((rc = run_stage<T, N>(c, c->uh, s, s == 1, s == 4))) return rc;
-    hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(1), 0, c->s, c->sc);
+        if ((rc = run_stage<T, N, S>(c, c->uh, b0, nb, s, s == 1, s == 4))) return rc;
+    hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(64), 0, c->s, c->sc + b0, nb);
     LAUNCHCHK(c);
     return NSG_OK;
 }
 
-template <int N> static int run_steps_N(nsg_ctx* c, long n) {
-    for (long i = 0; i < n; ++i) {
-        int rc = c->fp32 ? run_step<float, N>(c) : run_step<double, N>(c);
-        if (rc) return rc;
+// enqueue n steps of boxes [b0, b0+nb) on the batch's stream
+static int enqueue_steps(nsg_ctx* c, long n, int b0, int nb) {
+    return dispatch_NS(c->N, c->sched, [&](auto nn, auto ss) {
+        constexpr int NN = decltype(nn)::value, SS = decltype(ss)::value;
+        for (long i = 0; i < n; ++i) {
+            const int rc = c->fp32 ? run_step<float, NN, SS>(c, b0, nb) : run_step<double, NN, SS>(c, b0, nb);
+            if (rc) return rc;
+        }
+        return (int)NSG_OK;
+    });
+}
+
+// after steps: wait for the stream, then check every box's scalars (finite, under its limit)
+static int check_boxes(nsg_ctx* c) {
+    std::vector<DevScalars> hs(c->nbox);
+    HIPCHK(c, hipMemcpyAsync(hs.data(), c->sc, sizeof(DevScalars) * c->nbox, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    for (int b = 0; b < c->nbox; ++b) {
+        const char* box = c->nbox > 1 ? " in box " : "";
+        char bs[16] = "";
+        if (c->nbox > 1) snprintf(bs, sizeof bs, "%d", b);
+        if (!std::isfinite(hs[b].max_last) || !std::isfinite(hs[b].dt))
+            return fail(c, NSG_ERR_NONFINITE, "non-finite velocity%s%s after step %lld", box, bs, hs[b].steps);
+        if (c->max_speed[b] > 0.0 && hs[b].max_last > c->max_speed[b])
+            return fail(c, NSG_ERR_BLOWUP, "max|u| = %g exceeds the limit %g%s%s at step %lld", hs[b].max_last,
+                        c->max_speed[b], box, bs, hs[b].steps);
     }
     return NSG_OK;
 }
 
 // ------------------------------------------------------------------------------------------
-// C ABI (C linkage comes from the declarations in nsg.h)
+// batch creation
+// ------------------------------------------------------------------------------------------
+static void free_all(nsg_ctx* c) {
+    if (!c) return;
+    void* bufs[] = {c->uh, c->aux, c->K[0], c->K[1], c->K[2], c->B1, c->B2, c->twN64, c->twM64,
+                    c->twN32, c->twM32, c->sc, c->fslots, c->part};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    if (c->ev[0]) (void)hipEventDestroy(c->ev[0]);
+    if (c->ev[1]) (void)hipEventDestroy(c->ev[1]);
+    if (c->s) (void)hipStreamDestroy(c->s);
+}
+
+static const char* check_params(const nsg_params* p) {
+    const int N = p->N[0];
+    if (p->N[1] != N || p->N[2] != N) return "only cubic grids are supported (N[0]==N[1]==N[2])";
+    if (dispatch_N(N, [](auto) { return 0; }) != 0) return "unsupported N (8,12,16,24,32,48,64,96,128,256,512,1024)";
+    if (!(p->nu >= 0.0) || !(p->dt > 0.0)) return "nu must be >= 0 and dt > 0";
+    if (p->visc_pow < 1 || p->visc_pow > 4) return "visc_pow must be 1..4";
+    if (p->scheme != NSG_SCHEME_RK4CN && p->scheme != NSG_SCHEME_IFRK4) return "unknown scheme";
+    if (p->precision != NSG_FP64 && p->precision != NSG_FP32BULK) return "unknown precision";
+    if (p->force_eps > 0.0 && !(p->force_kf > 0.0 && p->force_kf < 7.5)) return "force_kf must be in (0, 7.5)";
+    return nullptr;
+}
+
+// the parameters every box of one batch must share
+static bool same_batch(const nsg_params& a, const nsg_params& b) {
+    return a.N[0] == b.N[0] && a.N[1] == b.N[1] && a.N[2] == b.N[2] && a.precision == b.precision &&
+           a.scheme == b.scheme && a.visc_pow == b.visc_pow && a.force_eps == b.force_eps &&
+           a.force_kf == b.force_kf && a.device == b.device;
+}
+
+// boxes p[0..nbox) share N, precision, scheme, visc_pow, forcing and device; nu, dt, t0, cfl and
+// max_speed may differ per box
+static int create_batch(const nsg_params* p, int nbox, nsg_ctx** out) {
+    if (!p || !out || nbox < 1) return NSG_ERR_INVALID;
+    *out = nullptr;
+    nsg_ctx* c = new (std::nothrow) nsg_ctx();
+    if (!c) return NSG_ERR_OOM;
+    *out = c;  // handed back on failure too, so the caller can read nsg_last_error, then destroy it
+    c->p = p[0];
+    for (int b = 0; b < nbox; ++b) {
+        if (const char* m = check_params(&p[b])) return fail(c, NSG_ERR_INVALID, "%s", m);
+        if (!same_batch(p[b], p[0]))
+            return fail(c, NSG_ERR_INVALID, "boxes of one batch must share N, precision, scheme, visc_pow, forcing, device");
+    }
+    const int N = p[0].N[0];
+    c->N = N;
+    c->nbox = nbox;
+    c->kmax = N / 3;
+    c->NK = 2 * c->kmax + 1;
+    c->NZP = nzp_of(N);
+    c->nspect = (int)std::sqrt(3.0 * (N / 2.0) * (N / 2.0)) + 1;
+    c->fp32 = p[0].precision == NSG_FP32BULK;
+    c->max_speed.resize(nbox);
+    for (int b = 0; b < nbox; ++b) {
+        c->max_speed[b] = p[b].max_speed;
+        if (p[b].cfl > 0.0) c->any_cfl = true;
+    }
+
+    hipError_t e = hipSetDevice(p[0].device);
+    if (e != hipSuccess) return fail(c, NSG_ERR_DEVICE, "hipSetDevice(%d): %s", p[0].device, hipGetErrorString(e));
+    HIPCHK(c, hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+    {
+        hipDeviceProp_t prop;
+        HIPCHK(c, hipGetDeviceProperties(&prop, p[0].device));
+        c->ncu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        const char* zb = std::getenv("NSG_ZBLOCKS_PER_CU");
+        if (zb && std::atoi(zb) > 0) c->zblocks_per_cu = std::atoi(zb);
+        const char* zo = std::getenv("NSG_ZOCC");
+        if (zo && std::atoi(zo) > 0) c->zocc = std::atoi(zo);
+    }
+    HIPCHK(c, hipEventCreate(&c->ev[0]));
+    HIPCHK(c, hipEventCreate(&c->ev[1]));
+
+    const size_t NK = c->NK, NZP = c->NZP, Nl = N, B = nbox;
+    c->sbox = (long long)(3 * NK * NK * NZP);
+    c->a1box = 6LL * N * xplane(c->NK, c->NZP);
+    c->a2box = 6LL * N * N * c->NZP;
+    c->a3box = 3LL * N * N * c->NZP;
+    c->a4box = 3LL * N * xplane(c->NK, c->NZP);
+    const size_t nstate = (size_t)c->sbox * B;
+    const size_t ebulk = c->fp32 ? sizeof(float2) : sizeof(double2);
+    // B1: A1 / A3 of every box (sized for FP64: the IC and real-space paths run in FP64), or one
+    //     box's real AoS field [N^3][3]
+    size_t b1 = (size_t)std::max(c->a1box, c->a3box) * B * sizeof(double2);
+    b1 = std::max(b1, 3 * Nl * Nl * Nl * sizeof(double));
+    // B2: A2 / A4 of every box, or the AoS spectral staging [N][N][N/2+1][3], or a real AoS field
+    size_t b2 = (size_t)std::max(c->a2box, c->a4box) * B * sizeof(double2);
+    b2 = std::max(b2, 3 * Nl * Nl * (Nl / 2 + 1) * sizeof(double2));
+    b2 = std::max(b2, 3 * Nl * Nl * Nl * sizeof(double));
+    auto alloc = [&](void** ptr, size_t bytes) -> int {
+        hipError_t ee = hipMalloc(ptr, bytes);
+        if (ee != hipSuccess) return fail(c, NSG_ERR_OOM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(ee));
+        return NSG_OK;
+    };
+    int rc;
+    if ((rc = alloc((void**)&c->uh, nstate * sizeof(double2)))) return rc;
+    if ((rc = alloc((void**)&c->aux, nstate * sizeof(double2)))) return rc;
+    for (int i = 0; i < 3; ++i)
+        if ((rc = alloc(&c->K[i], nstate * ebulk))) return rc;
+    if ((rc = alloc(&c->B1, b1))) return rc;
+    if ((rc = alloc(&c->B2, b2))) return rc;
+    if ((rc = alloc((void**)&c->sc, sizeof(DevScalars) * B))) return rc;
+    if ((rc = alloc((void**)&c->fslots, sizeof(double) * NK * B))) return rc;
+    c->part_n = std::max<size_t>(7 * (size_t)c->nspect, ((size_t)c->sbox / 3 + 255) / 256 + 1);
+    if ((rc = alloc((void**)&c->part, sizeof(double) * c->part_n))) return rc;
+    if ((rc = upload_table(c, N, &c->twN64, &c->twN32))) return rc;
+    if ((rc = upload_table(c, N / 2, &c->twM64, &c->twM32))) return rc;
+    HIPCHK(c, hipMemset(c->uh, 0, nstate * sizeof(double2)));
+    HIPCHK(c, hipMemset(c->fslots, 0, sizeof(double) * NK * B));
+    for (int i = 0; i < 3; ++i) HIPCHK(c, hipMemset(c->K[i], 0, nstate * ebulk));
+    std::vector<DevScalars> hs(B);
+    for (int b = 0; b < nbox; ++b) {
+        hs[b] = DevScalars{};
+        hs[b].dt = p[b].dt; hs[b].t = p[b].t0; hs[b].maxbits = 0; hs[b].max_last = 0.0; hs[b].steps = 0;
+        hs[b].cfl = p[b].cfl; hs[b].dx = 2.0 * M_PI / N; hs[b].nu = p[b].nu;
+    }
+    HIPCHK(c, hipMemcpy(c->sc, hs.data(), sizeof(DevScalars) * B, hipMemcpyHostToDevice));
+    HIPCHK(c, hipDeviceSynchronize());  // the null-stream memsets above must land before work on c->s
+    return NSG_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// per-box operations (box b of a batch); each synchronises the batch's stream before returning
+// ------------------------------------------------------------------------------------------
+static size_t aos_spectral_bytes(const nsg_ctx* c) { return 3 * (size_t)c->N * c->N * (c->N / 2 + 1) * sizeof(double2); }
+static size_t aos_real_bytes(const nsg_ctx* c) { return 3 * (size_t)c->N * c->N * c->N * sizeof(double); }
+
+// host u_hat [N][N][N/2+1][3] -> state box b of `dst` (masked to the 2/3 sphere)
+static int set_state_box(nsg_ctx* c, int b, const double* u_hat, double2* dst) {
+    HIPCHK(c, hipMemcpyAsync(c->B2, u_hat, aos_spectral_bytes(c), hipMemcpyHostToDevice, c->s));
+    hipLaunchKernelGGL(aos_to_state_kernel, dim3(2048), dim3(256), 0, c->s, (const double2*)c->B2, dst + b * c->sbox,
+                       c->N, c->kmax, c->NK, c->NZP);
+    LAUNCHCHK(c);
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return NSG_OK;
+}
+
+// one box's compact array (state: FP64; K: bulk precision) -> host AoS [N][N][N/2+1][3]
+static int download_compact(nsg_ctx* c, const void* src, bool bulk, double* out) {
+    if (bulk && c->fp32)
+        hipLaunchKernelGGL(compact_to_aos_kernel<float2>, dim3(2048), dim3(256), 0, c->s, (const float2*)src, (double2*)c->B2, c->N, c->kmax, c->NK, c->NZP);
+    else
+        hipLaunchKernelGGL(compact_to_aos_kernel<double2>, dim3(2048), dim3(256), 0, c->s, (const double2*)src, (double2*)c->B2, c->N, c->kmax, c->NK, c->NZP);
+    LAUNCHCHK(c);
+    HIPCHK(c, hipMemcpyAsync(out, c->B2, aos_spectral_bytes(c), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return NSG_OK;
+}
+
+// r2c of a real AoS field already in B2 into state box b (FP64, schedule 0).  A3/A4 of box b are
+// the scratch; the A4 region may overlap the AoS input, which pass 3 has consumed by then.
+template <int N> static int r2c_into_state(nsg_ctx* c, int b) {
+    int rc;
+    if ((rc = launch_passz<double, N, 0, ZM_R2C>(c, c->B2, (double2*)c->B1 + b * c->a3box, 0, 0, 1, c->sc + b, 1.0, 0)))
+        return rc;
+    if ((rc = launch_passy<double, N, 0, -1>(c, b, 1, 3))) return rc;
+    return launch_pass5<double, N, 0, EP_STORE>(c, c->uh, b, 1, 1, nullptr, 0, 1.0);
+}
+
+static int init_state_box(nsg_ctx* c, int b, int kind, double k0, double energy, uint64_t seed) {
+    if (kind == NSG_IC_TAYLOR_GREEN || kind == NSG_IC_SHAPIRO) {
+        hipLaunchKernelGGL(ic_real_kernel, dim3(4096), dim3(256), 0, c->s, (double*)c->B2, c->N,
+                           kind == NSG_IC_TAYLOR_GREEN ? 0 : 1, c->p.nu, 0.0);
+        LAUNCHCHK(c);
+        const int rc = dispatch_N(c->N, [&](auto n) { return r2c_into_state<decltype(n)::value>(c, b); });
+        if (rc) return rc;
+    } else if (kind == NSG_IC_K4) {
+        if (!(k0 > 0.0) || !(energy >= 0.0)) return fail(c, NSG_ERR_INVALID, "k4 IC needs k0 > 0 and energy >= 0");
+        const long long plane = (long long)c->NK * c->NK * c->NZP;
+        const unsigned nb = (unsigned)((plane + 255) / 256);
+        if ((size_t)nb + 1 > c->part_n) return fail(c, NSG_ERR_INVALID, "internal: partial buffer too small");
+        double2* uh = c->uh + b * c->sbox;
+        hipLaunchKernelGGL(ic_k4_kernel, dim3(nb), dim3(256), 0, c->s, uh, c->part, c->N, c->kmax, c->NK, c->NZP, k0,
+                           (unsigned long long)seed);
+        LAUNCHCHK(c);
+        hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(1), 0, c->s, c->part, (int)nb, c->part + nb);
+        LAUNCHCHK(c);
+        const double N3 = (double)c->N * c->N * c->N;
+        hipLaunchKernelGGL(scale_state_kernel, dim3(2048), dim3(256), 0, c->s, uh, 3 * plane, (const double*)(c->part + nb), energy, N3);
+        LAUNCHCHK(c);
+    } else {
+        return fail(c, NSG_ERR_INVALID, "unknown initial condition %d", kind);
+    }
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return NSG_OK;
+}
+
+static int measures_box(nsg_ctx* c, int b, nsg_totals* out, double* E_k, double* Ens_k, int n_spect, int quirk) {
+    if ((E_k || Ens_k) && n_spect < c->nspect) return fail(c, NSG_ERR_INVALID, "n_spect %d < %d", n_spect, c->nspect);
+    const int ns = c->nspect;
+    hipLaunchKernelGGL(measures_kernel, dim3(ns, 1), dim3(256), 0, c->s, c->uh + b * c->sbox, c->part, c->N, c->kmax,
+                       c->NK, c->NZP, (const DevScalars*)(c->sc + b), c->p.visc_pow, quirk ? 1 : 0, c->sbox);
+    LAUNCHCHK(c);
+    std::vector<double> h((size_t)ns * 7);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    const double N3 = (double)c->N * c->N * c->N;
+    const double nf = 0.5 / (N3 * N3), cf = 8.0 * M_PI * M_PI * M_PI;
+    double t[5] = {0, 0, 0, 0, 0};
+    for (int q = 0; q < ns; ++q)
+        for (int j = 0; j < 5; ++j) t[j] += h[(size_t)q * 7 + j];
+    out->energy = t[0] * cf * nf;
+    out->enstrophy = t[1] * cf * nf;
+    out->palinstrophy = t[2] * cf * nf;
+    out->helicity = t[3] * cf * nf;
+    out->dissipation = t[4] * 2.0 * cf * nf;
+    for (int q = 0; q < n_spect && (E_k || Ens_k); ++q) {
+        if (E_k) E_k[q] = q < ns ? cf * nf * h[(size_t)q * 7 + 5] : 0.0;
+        if (Ens_k) Ens_k[q] = q < ns ? cf * nf * h[(size_t)q * 7 + 6] : 0.0;
+    }
+    const double* vals[] = {&out->energy, &out->enstrophy, &out->palinstrophy, &out->helicity, &out->dissipation};
+    for (const double* v : vals)
+        if (!std::isfinite(*v)) return fail(c, NSG_ERR_NONFINITE, "non-finite diagnostics");
+    return NSG_OK;
+}
+
+// real-space u (which 0) or w (which 1) of box b as AoS [N^3][3] doubles at the start of B1
+template <int N> static int realspace_N(nsg_ctx* c, int b, int which) {
+    int rc;
+    if ((rc = launch_pass1<double, N, 0>(c, c->uh, b, 1, nullptr, 1, which == 0 ? 1 : 2, 0))) return rc;
+    // pass 2 on three fields: A1 [3][N][NK][NZP] -> A2 [3][N][N][NZP]
+    if ((rc = launch_passy<double, N, 0, +1>(c, b, 1, 3))) return rc;
+    const double N3 = (double)N * N * N;
+    return launch_passz<double, N, 0, ZM_REAL>(c, (const double2*)c->B2 + b * c->a2box, c->B1, 0, 0, 1, c->sc + b,
+                                               1.0 / N3, 0);
+}
+
+static int realspace_box(nsg_ctx* c, int b, int which, double* out) {
+    const int rc = dispatch_N(c->N, [&](auto n) { return realspace_N<decltype(n)::value>(c, b, which); });
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(out, c->B1, aos_real_bytes(c), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return NSG_OK;
+}
+
+static int get_time_box(nsg_ctx* c, int b, double* t, double* dt, double* max_speed) {
+    DevScalars hs;
+    HIPCHK(c, hipMemcpyAsync(&hs, c->sc + b, sizeof hs, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    if (t) *t = hs.t;
+    if (dt) *dt = hs.dt;
+    if (max_speed) *max_speed = hs.max_last;
+    return NSG_OK;
+}
+
+static int set_time_box(nsg_ctx* c, int b, double t, double dt) {
+    DevScalars hs;
+    HIPCHK(c, hipMemcpyAsync(&hs, c->sc + b, sizeof hs, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    hs.t = t;
+    hs.dt = dt;
+    HIPCHK(c, hipMemcpyAsync(c->sc + b, &hs, sizeof hs, hipMemcpyHostToDevice, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return NSG_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// C ABI, single box (C linkage comes from the declarations in nsg.h)
 // ------------------------------------------------------------------------------------------
 
 void nsg_default_params(nsg_params* p) {
@@ -355,17 +694,6 @@ const char* nsg_last_error(const nsg_ctx* ctx) { return ctx ? ctx->err.c_str() :
 
 int nsg_n_spect(const nsg_ctx* ctx) { return ctx ? ctx->nspect : 0; }
 
-static void free_all(nsg_ctx* c) {
-    if (!c) return;
-    void* bufs[] = {c->uh, c->aux, c->K[0], c->K[1], c->K[2], c->B1, c->B2, c->twN64, c->twM64,
-                    c->twN32, c->twM32, c->sc, c->fslots, c->part};
-    for (void* b : bufs)
-        if (b) (void)hipFree(b);
-    if (c->ev[0]) (void)hipEventDestroy(c->ev[0]);
-    if (c->ev[1]) (void)hipEventDestroy(c->ev[1]);
-    if (c->s) (void)hipStreamDestroy(c->s);
-}
-
 void nsg_destroy(nsg_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->p.device);
@@ -374,111 +702,12 @@ void nsg_destroy(nsg_ctx* ctx) {
     delete ctx;
 }
 
-static thread_local std::string g_create_err;
-
-int nsg_create(const nsg_params* p, nsg_ctx** out) {
-    if (!p || !out) return NSG_ERR_INVALID;
-    *out = nullptr;
-    nsg_ctx* c = new (std::nothrow) nsg_ctx();
-    if (!c) return NSG_ERR_OOM;
-    c->p = *p;
-    const int N = p->N[0];
-    auto bad = [&](const char* msg) {
-        g_create_err = msg;
-        c->err = msg;
-        *out = c;  // hand back the context so the caller can read nsg_last_error, then destroy it
-        return NSG_ERR_INVALID;
-    };
-    if (p->N[1] != N || p->N[2] != N) return bad("only cubic grids are supported (N[0]==N[1]==N[2])");
-    if (dispatch_N(N, [](auto) { return 0; }) != 0) return bad("unsupported N (8,12,16,24,32,48,64,96,128,256,512,1024)");
-    if (!(p->nu >= 0.0) || !(p->dt > 0.0)) return bad("nu must be >= 0 and dt > 0");
-    if (p->visc_pow < 1 || p->visc_pow > 4) return bad("visc_pow must be 1..4");
-    if (p->scheme != NSG_SCHEME_RK4CN && p->scheme != NSG_SCHEME_IFRK4) return bad("unknown scheme");
-    if (p->precision != NSG_FP64 && p->precision != NSG_FP32BULK) return bad("unknown precision");
-    if (p->force_eps > 0.0 && !(p->force_kf > 0.0 && p->force_kf < 7.5)) return bad("force_kf must be in (0, 7.5)");
-    c->N = N;
-    c->kmax = N / 3;
-    c->NK = 2 * c->kmax + 1;
-    c->NZP = ((c->kmax + 1 + 15) / 16) * 16;
-    c->nspect = (int)std::sqrt(3.0 * (N / 2.0) * (N / 2.0)) + 1;
-    c->fp32 = p->precision == NSG_FP32BULK;
-
-    hipError_t e = hipSetDevice(p->device);
-    if (e != hipSuccess) { *out = c; return fail(c, NSG_ERR_DEVICE, "hipSetDevice(%d): %s", p->device, hipGetErrorString(e)); }
-    *out = c;
-    HIPCHK(c, hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
-    {
-        hipDeviceProp_t prop;
-        HIPCHK(c, hipGetDeviceProperties(&prop, p->device));
-        c->ncu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
-        const char* zb = std::getenv("NSG_ZBLOCKS_PER_CU");
-        if (zb && std::atoi(zb) > 0) c->zblocks_per_cu = std::atoi(zb);
-        const char* zo = std::getenv("NSG_ZOCC");
-        if (zo && std::atoi(zo) > 0) c->zocc = std::atoi(zo);
-    }
-    HIPCHK(c, hipEventCreate(&c->ev[0]));
-    HIPCHK(c, hipEventCreate(&c->ev[1]));
-
-    const size_t NK = c->NK, NZP = c->NZP, Nl = N;
-    const size_t nstate = 3 * NK * NK * NZP;
-    const size_t ebulk = c->fp32 ? sizeof(float2) : sizeof(double2);
-    // B1: A1 [6][N][NK][NZP], A3 [3][N][N][NZP], real AoS [N^3][3]  (FP64 sized: IC/realspace paths run FP64)
-    size_t b1 = std::max(6 * Nl * (size_t)xplane(c->NK, c->NZP), 3 * Nl * Nl * NZP) * sizeof(double2);
-    b1 = std::max(b1, 3 * Nl * Nl * Nl * sizeof(double));
-    // B2: A2 [6][N][N][NZP], A4, full AoS spectral staging [N][N][N/2+1][3], real AoS
-    size_t b2 = std::max(6 * Nl * Nl * NZP, 3 * Nl * (size_t)xplane(c->NK, c->NZP)) * sizeof(double2);
-    b2 = std::max(b2, 3 * Nl * Nl * (Nl / 2 + 1) * sizeof(double2));
-    b2 = std::max(b2, 3 * Nl * Nl * Nl * sizeof(double));
-    auto alloc = [&](void** ptr, size_t bytes) -> int {
-        hipError_t ee = hipMalloc(ptr, bytes);
-        if (ee != hipSuccess) return fail(c, NSG_ERR_OOM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(ee));
-        return NSG_OK;
-    };
-    int rc;
-    if ((rc = alloc((void**)&c->uh, nstate * sizeof(double2)))) return rc;
-    if ((rc = alloc((void**)&c->aux, nstate * sizeof(double2)))) return rc;
-    for (int i = 0; i < 3; ++i)
-        if ((rc = alloc(&c->K[i], nstate * ebulk))) return rc;
-    if ((rc = alloc(&c->B1, b1))) return rc;
-    if ((rc = alloc(&c->B2, b2))) return rc;
-    if ((rc = alloc((void**)&c->sc, sizeof(DevScalars)))) return rc;
-    if ((rc = alloc((void**)&c->fslots, sizeof(double) * NK))) return rc;
-    c->part_n = std::max<size_t>(7 * (size_t)c->nspect, (nstate / 3 + 255) / 256 + 1);
-    if ((rc = alloc((void**)&c->part, sizeof(double) * c->part_n))) return rc;
-    if ((rc = upload_table(c, N, &c->twN64, &c->twN32))) return rc;
-    if ((rc = upload_table(c, N / 2, &c->twM64, &c->twM32))) return rc;
-    HIPCHK(c, hipMemset(c->uh, 0, nstate * sizeof(double2)));
-    HIPCHK(c, hipMemset(c->fslots, 0, sizeof(double) * NK));
-    for (int i = 0; i < 3; ++i) HIPCHK(c, hipMemset(c->K[i], 0, nstate * ebulk));
-    DevScalars hs{};
-    hs.dt = p->dt; hs.t = p->t0; hs.maxbits = 0; hs.max_last = 0.0; hs.steps = 0;
-    hs.cfl = p->cfl; hs.dx = 2.0 * M_PI / N;
-    HIPCHK(c, hipMemcpy(c->sc, &hs, sizeof hs, hipMemcpyHostToDevice));
-    HIPCHK(c, hipDeviceSynchronize());  // the null-stream memsets above must land before work on c->s
-    return NSG_OK;
-}
+int nsg_create(const nsg_params* p, nsg_ctx** out) { return create_batch(p, 1, out); }
 
 int nsg_set_state(nsg_ctx* c, const double* u_hat) {
     if (!c || !u_hat) return NSG_ERR_INVALID;
     HIPCHK(c, hipSetDevice(c->p.device));
-    const size_t full = 3 * (size_t)c->N * c->N * (c->N / 2 + 1) * sizeof(double2);
-    HIPCHK(c, hipMemcpyAsync(c->B2, u_hat, full, hipMemcpyHostToDevice, c->s));
-    hipLaunchKernelGGL(aos_to_state_kernel, dim3(2048), dim3(256), 0, c->s, (const double2*)c->B2, c->uh, c->N, c->kmax, c->NK, c->NZP);
-    LAUNCHCHK(c);
-    HIPCHK(c, hipStreamSynchronize(c->s));
-    return NSG_OK;
-}
-
-static int download_compact(nsg_ctx* c, const void* src, bool bulk, double* out) {
-    const size_t full = 3 * (size_t)c->N * c->N * (c->N / 2 + 1) * sizeof(double2);
-    if (bulk && c->fp32)
-        hipLaunchKernelGGL(compact_to_aos_kernel<float2>, dim3(2048), dim3(256), 0, c->s, (const float2*)src, (double2*)c->B2, c->N, c->kmax, c->NK, c->NZP);
-    else
-        hipLaunchKernelGGL(compact_to_aos_kernel<double2>, dim3(2048), dim3(256), 0, c->s, (const double2*)src, (double2*)c->B2, c->N, c->kmax, c->NK, c->NZP);
-    LAUNCHCHK(c);
-    HIPCHK(c, hipMemcpyAsync(out, c->B2, full, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(c, hipStreamSynchronize(c->s));
-    return NSG_OK;
+    return set_state_box(c, 0, u_hat, c->uh);
 }
 
 int nsg_get_state(nsg_ctx* c, double* u_hat) {
@@ -487,20 +716,11 @@ int nsg_get_state(nsg_ctx* c, double* u_hat) {
     return download_compact(c, c->uh, false, u_hat);
 }
 
-// r2c of a real AoS field already in B2 into the state (FP64 path)
-template <int N> static int r2c_into_state(nsg_ctx* c) {
-    int rc;
-    if ((rc = launch_passz<double, N, ZM_R2C>(c, c->B2, c->B1, 1.0, 0))) return rc;
-    if ((rc = launch_passy<double, N, -1>(c, c->B1, c->B2, 3))) return rc;
-    return launch_pass5<double, N, EP_STORE>(c, c->uh, 1, nullptr, 0, 1.0);
-}
-
 int nsg_set_state_real(nsg_ctx* c, const double* u) {
     if (!c || !u) return NSG_ERR_INVALID;
     HIPCHK(c, hipSetDevice(c->p.device));
-    const size_t bytes = 3 * (size_t)c->N * c->N * c->N * sizeof(double);
-    HIPCHK(c, hipMemcpyAsync(c->B2, u, bytes, hipMemcpyHostToDevice, c->s));
-    int rc = dispatch_N(c->N, [&](auto n) { return r2c_into_state<decltype(n)::value>(c); });
+    HIPCHK(c, hipMemcpyAsync(c->B2, u, aos_real_bytes(c), hipMemcpyHostToDevice, c->s));
+    const int rc = dispatch_N(c->N, [&](auto n) { return r2c_into_state<decltype(n)::value>(c, 0); });
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->s));
     return NSG_OK;
@@ -509,41 +729,18 @@ int nsg_set_state_real(nsg_ctx* c, const double* u) {
 int nsg_init_state(nsg_ctx* c, int kind, double k0, double energy, uint64_t seed) {
     if (!c) return NSG_ERR_INVALID;
     HIPCHK(c, hipSetDevice(c->p.device));
-    if (kind == NSG_IC_TAYLOR_GREEN || kind == NSG_IC_SHAPIRO) {
-        hipLaunchKernelGGL(ic_real_kernel, dim3(4096), dim3(256), 0, c->s, (double*)c->B2, c->N,
-                           kind == NSG_IC_TAYLOR_GREEN ? 0 : 1, c->p.nu, 0.0);
-        LAUNCHCHK(c);
-        int rc = dispatch_N(c->N, [&](auto n) { return r2c_into_state<decltype(n)::value>(c); });
-        if (rc) return rc;
-    } else if (kind == NSG_IC_K4) {
-        if (!(k0 > 0.0) || !(energy >= 0.0)) return fail(c, NSG_ERR_INVALID, "k4 IC needs k0 > 0 and energy >= 0");
-        const long long plane = (long long)c->NK * c->NK * c->NZP;
-        const unsigned nb = (unsigned)((plane + 255) / 256);
-        if ((size_t)nb + 1 > c->part_n) return fail(c, NSG_ERR_INVALID, "internal: partial buffer too small");
-        hipLaunchKernelGGL(ic_k4_kernel, dim3(nb), dim3(256), 0, c->s, c->uh, c->part, c->N, c->kmax, c->NK, c->NZP, k0, (unsigned long long)seed);
-        LAUNCHCHK(c);
-        hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(1), 0, c->s, c->part, (int)nb, c->part + nb);
-        LAUNCHCHK(c);
-        const double N3 = (double)c->N * c->N * c->N;
-        hipLaunchKernelGGL(scale_state_kernel, dim3(2048), dim3(256), 0, c->s, c->uh, 3 * plane, (const double*)(c->part + nb), energy, N3);
-        LAUNCHCHK(c);
-    } else {
-        return fail(c, NSG_ERR_INVALID, "unknown initial condition %d", kind);
-    }
-    HIPCHK(c, hipStreamSynchronize(c->s));
-    return NSG_OK;
+    return init_state_box(c, 0, kind, k0, energy, seed);
 }
 
 int nsg_rhs(nsg_ctx* c, const double* u_hat, double* out) {
     if (!c || !u_hat || !out) return NSG_ERR_INVALID;
     HIPCHK(c, hipSetDevice(c->p.device));
-    const size_t full = 3 * (size_t)c->N * c->N * (c->N / 2 + 1) * sizeof(double2);
-    HIPCHK(c, hipMemcpyAsync(c->B2, u_hat, full, hipMemcpyHostToDevice, c->s));
-    hipLaunchKernelGGL(aos_to_state_kernel, dim3(2048), dim3(256), 0, c->s, (const double2*)c->B2, c->aux, c->N, c->kmax, c->NK, c->NZP);
-    LAUNCHCHK(c);
-    int rc = dispatch_N(c->N, [&](auto n) {
-        constexpr int NN = decltype(n)::value;
-        return c->fp32 ? run_stage<float, NN>(c, c->aux, 1, false, false) : run_stage<double, NN>(c, c->aux, 1, false, false);
+    int rc = set_state_box(c, 0, u_hat, c->aux);
+    if (rc) return rc;
+    rc = dispatch_NS(c->N, c->sched, [&](auto nn, auto ss) {
+        constexpr int NN = decltype(nn)::value, SS = decltype(ss)::value;
+        return c->fp32 ? run_stage<float, NN, SS>(c, c->aux, 0, 1, 1, false, false)
+                       : run_stage<double, NN, SS>(c, c->aux, 0, 1, 1, false, false);
     });
     if (rc) return rc;
     return download_compact(c, c->K[0], true, out);
@@ -552,97 +749,40 @@ int nsg_rhs(nsg_ctx* c, const double* u_hat, double* out) {
 int nsg_step(nsg_ctx* c, long nsteps) {
     if (!c || nsteps < 0) return NSG_ERR_INVALID;
     HIPCHK(c, hipSetDevice(c->p.device));
-    int rc = dispatch_N(c->N, [&](auto n) { return run_steps_N<decltype(n)::value>(c, nsteps); });
+    const int rc = enqueue_steps(c, nsteps, 0, c->nbox);
     if (rc) return rc;
-    DevScalars hs;
-    HIPCHK(c, hipMemcpyAsync(&hs, c->sc, sizeof hs, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(c, hipStreamSynchronize(c->s));
-    if (nsteps > 0) {
-        if (!std::isfinite(hs.max_last) || !std::isfinite(hs.dt))
-            return fail(c, NSG_ERR_NONFINITE, "non-finite velocity after step %lld", hs.steps);
-        if (c->p.max_speed > 0.0 && hs.max_last > c->p.max_speed)
-            return fail(c, NSG_ERR_BLOWUP, "max|u| = %g exceeds the limit %g at step %lld", hs.max_last, c->p.max_speed, hs.steps);
+    if (nsteps == 0) {
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        return NSG_OK;
     }
-    return NSG_OK;
+    return check_boxes(c);
 }
 
 int nsg_measures(nsg_ctx* c, nsg_totals* out, double* E_k, double* Ens_k, int n_spect, int quirk) {
     if (!c || !out) return NSG_ERR_INVALID;
-    if ((E_k || Ens_k) && n_spect < c->nspect) return fail(c, NSG_ERR_INVALID, "n_spect %d < %d", n_spect, c->nspect);
     HIPCHK(c, hipSetDevice(c->p.device));
-    const int ns = c->nspect;
-    hipLaunchKernelGGL(measures_kernel, dim3(ns), dim3(256), 0, c->s, c->uh, c->part, c->N, c->kmax, c->NK, c->NZP,
-                       c->p.nu, c->p.visc_pow, quirk ? 1 : 0);
-    LAUNCHCHK(c);
-    std::vector<double> h((size_t)ns * 7);
-    HIPCHK(c, hipMemcpyAsync(h.data(), c->part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->s));
-    HIPCHK(c, hipStreamSynchronize(c->s));
-    const double N3 = (double)c->N * c->N * c->N;
-    const double nf = 0.5 / (N3 * N3), cf = 8.0 * M_PI * M_PI * M_PI;
-    double t[5] = {0, 0, 0, 0, 0};
-    for (int b = 0; b < ns; ++b)
-        for (int j = 0; j < 5; ++j) t[j] += h[(size_t)b * 7 + j];
-    out->energy = t[0] * cf * nf;
-    out->enstrophy = t[1] * cf * nf;
-    out->palinstrophy = t[2] * cf * nf;
-    out->helicity = t[3] * cf * nf;
-    out->dissipation = t[4] * 2.0 * cf * nf;
-    for (int b = 0; b < n_spect && (E_k || Ens_k); ++b) {
-        if (E_k) E_k[b] = b < ns ? cf * nf * h[(size_t)b * 7 + 5] : 0.0;
-        if (Ens_k) Ens_k[b] = b < ns ? cf * nf * h[(size_t)b * 7 + 6] : 0.0;
-    }
-    const double* vals[] = {&out->energy, &out->enstrophy, &out->palinstrophy, &out->helicity, &out->dissipation};
-    for (const double* v : vals)
-        if (!std::isfinite(*v)) return fail(c, NSG_ERR_NONFINITE, "non-finite diagnostics");
-    return NSG_OK;
-}
-
-template <int N> static int realspace_N(nsg_ctx* c, int which) {
-    int rc;
-    if ((rc = launch_pass1<double, N>(c, c->uh, nullptr, 1, which == 0 ? 1 : 2, 0))) return rc;
-    // pass 2 on three fields: A1 [3][N][NK][NZP] -> A2 [3][N][N][NZP]
-    if ((rc = launch_passy<double, N, +1>(c, c->B1, c->B2, 3))) return rc;
-    const double N3 = (double)N * N * N;
-    return launch_passz<double, N, ZM_REAL>(c, c->B2, c->B1, 1.0 / N3, 0);
+    return measures_box(c, 0, out, E_k, Ens_k, n_spect, quirk);
 }
 
 int nsg_realspace(nsg_ctx* c, int which, double* out) {
     if (!c || !out || (which != 0 && which != 1)) return NSG_ERR_INVALID;
     HIPCHK(c, hipSetDevice(c->p.device));
-    int rc = dispatch_N(c->N, [&](auto n) { return realspace_N<decltype(n)::value>(c, which); });
-    if (rc) return rc;
-    const size_t bytes = 3 * (size_t)c->N * c->N * c->N * sizeof(double);
-    HIPCHK(c, hipMemcpyAsync(out, c->B1, bytes, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(c, hipStreamSynchronize(c->s));
-    return NSG_OK;
+    return realspace_box(c, 0, which, out);
 }
 
 int nsg_get_time(nsg_ctx* c, double* t, double* dt, double* max_speed) {
     if (!c) return NSG_ERR_INVALID;
     HIPCHK(c, hipSetDevice(c->p.device));
-    DevScalars hs;
-    HIPCHK(c, hipMemcpyAsync(&hs, c->sc, sizeof hs, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(c, hipStreamSynchronize(c->s));
-    if (t) *t = hs.t;
-    if (dt) *dt = hs.dt;
-    if (max_speed) *max_speed = hs.max_last;
-    return NSG_OK;
+    return get_time_box(c, 0, t, dt, max_speed);
 }
 
 int nsg_set_time(nsg_ctx* c, double t, double dt) {
     if (!c || !(dt > 0.0)) return NSG_ERR_INVALID;
     HIPCHK(c, hipSetDevice(c->p.device));
-    DevScalars hs;
-    HIPCHK(c, hipMemcpyAsync(&hs, c->sc, sizeof hs, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(c, hipStreamSynchronize(c->s));
-    hs.t = t;
-    hs.dt = dt;
-    HIPCHK(c, hipMemcpyAsync(c->sc, &hs, sizeof hs, hipMemcpyHostToDevice, c->s));
-    HIPCHK(c, hipStreamSynchronize(c->s));
-    return NSG_OK;
+    return set_time_box(c, 0, t, dt);
 }
 
-static void pass_bytes(const nsg_ctx* c, double* b /*[5]: stage-averaged*/) {
+static void pass_bytes(const nsg_ctx* c, double* b /*[5]: stage-averaged, one box*/) {
     const double N = c->N, NK = c->NK, NZL = c->kmax + 1;
     const double E = c->fp32 ? 8.0 : 16.0, S = 16.0;
     const double st = 3 * NK * NK * NZL;  // complex elements of a 3-vector state
@@ -659,7 +799,7 @@ double nsg_step_bytes(const nsg_ctx* c) {
     if (!c) return 0.0;
     double b[5];
     pass_bytes(c, b);
-    return 4.0 * (b[0] + b[1] + b[2] + b[3] + b[4]);
+    return 4.0 * (b[0] + b[1] + b[2] + b[3] + b[4]) * c->nbox;
 }
 
 int nsg_profile_passes(nsg_ctx* c, int iters, double* ms, double* bytes) {
@@ -668,7 +808,7 @@ int nsg_profile_passes(nsg_ctx* c, int iters, double* ms, double* bytes) {
     for (int k = 0; k < 5; ++k) { c->prof_ms[k] = 0.0; c->prof_cnt[k] = 0; }
     HIPCHK(c, hipStreamSynchronize(c->s));
     c->prof = true;
-    int rc = dispatch_N(c->N, [&](auto n) { return run_steps_N<decltype(n)::value>(c, iters); });
+    const int rc = enqueue_steps(c, iters, 0, c->nbox);
     c->prof = false;
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->s));
@@ -678,6 +818,349 @@ int nsg_profile_passes(nsg_ctx* c, int iters, double* ms, double* bytes) {
         tot += c->prof_ms[k];
     }
     ms[5] = tot / iters;
-    if (bytes) pass_bytes(c, bytes);
+    if (bytes) {
+        pass_bytes(c, bytes);
+        for (int k = 0; k < 5; ++k) bytes[k] *= c->nbox;
+    }
     return NSG_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// Schedules and the autotuner: every schedule of the batch's size is timed on the batch's own
+// state (saved in aux and restored after each trial, scalars too) and the fastest is kept.
+// Choices are cached per (N, precision, nbox) for the life of the process.
+// ------------------------------------------------------------------------------------------
+int nsg_schedule_count(int N) {
+    const int n = dispatch_N(N, [](auto nn) { return nsched(decltype(nn)::value); });
+    return n > 0 ? n : 0;
+}
+
+int nsg_set_schedule(nsg_ctx* c, int sched) {
+    if (!c) return NSG_ERR_INVALID;
+    const int ns = nsg_schedule_count(c->N);
+    if (sched < 0 || sched >= ns) return fail(c, NSG_ERR_INVALID, "schedule %d out of range [0,%d) for N=%d", sched, ns, c->N);
+    c->sched = sched;
+    return NSG_OK;
+}
+
+static std::mutex g_tune_mu;
+static std::map<std::tuple<int, int, int>, int> g_tune_cache;
+
+static int autotune_ctx(nsg_ctx* c, int iters, int use_cache, int* chosen, double* ms_per_step) {
+    const auto key = std::make_tuple(c->N, (int)c->fp32, c->nbox);
+    const int ns = nsg_schedule_count(c->N);
+    if (use_cache) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            c->sched = it->second;
+            if (chosen) *chosen = c->sched;
+            return NSG_OK;
+        }
+    }
+    const size_t sbytes = (size_t)c->sbox * c->nbox * sizeof(double2);
+    std::vector<DevScalars> hs(c->nbox);
+    HIPCHK(c, hipMemcpyAsync(c->aux, c->uh, sbytes, hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(c, hipMemcpyAsync(hs.data(), c->sc, sizeof(DevScalars) * c->nbox, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    int best = 0;
+    double best_ms = 1e300;
+    for (int S = 0; S < ns; ++S) {
+        c->sched = S;
+        int rc = enqueue_steps(c, 1, 0, c->nbox);  // untimed: first launches load the code objects
+        if (rc) return rc;
+        HIPCHK(c, hipEventRecord(c->ev[0], c->s));
+        if ((rc = enqueue_steps(c, iters, 0, c->nbox))) return rc;
+        HIPCHK(c, hipEventRecord(c->ev[1], c->s));
+        HIPCHK(c, hipEventSynchronize(c->ev[1]));
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+        const double per = ms / iters;
+        if (ms_per_step) ms_per_step[S] = per;
+        if (per < best_ms) { best_ms = per; best = S; }
+        HIPCHK(c, hipMemcpyAsync(c->uh, c->aux, sbytes, hipMemcpyDeviceToDevice, c->s));
+        HIPCHK(c, hipMemcpyAsync(c->sc, hs.data(), sizeof(DevScalars) * c->nbox, hipMemcpyHostToDevice, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+    }
+    c->sched = best;
+    if (chosen) *chosen = best;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = best;
+    return NSG_OK;
+}
+
+int nsg_autotune(nsg_ctx* c, int iters, int use_cache, int* chosen, double* ms_per_step) {
+    if (!c || iters <= 0) return NSG_ERR_INVALID;
+    HIPCHK(c, hipSetDevice(c->p.device));
+    return autotune_ctx(c, iters, use_cache, chosen, ms_per_step);
+}
+
+// ------------------------------------------------------------------------------------------
+// Ensembles: boxes grouped into batches (one per set of shared parameters, in order of first
+// appearance); one step of every batch is captured once into a HIP graph -- the batches fork
+// from one stream onto their own streams and join back -- and nsg_ensemble_step replays it.
+// ------------------------------------------------------------------------------------------
+struct nsg_ensemble {
+    std::vector<nsg_ctx*> groups;
+    std::vector<std::pair<int, int>> where;  // box -> (group, index in the group)
+    hipStream_t master = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    std::string err;
+    int device = 0;
+};
+
+static int efail(nsg_ensemble* e, int code, const std::string& msg) {
+    if (e) e->err = msg;
+    return code;
+}
+static int egroup(nsg_ensemble* e, const nsg_ctx* g, int rc) { return efail(e, rc, g ? g->err : std::string("group error")); }
+
+#define EHIP(e, expr)                                                                                  \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess) return efail((e), NSG_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int nsg_ensemble_create(const nsg_params* boxes, int nbox, nsg_ensemble** out) {
+    if (!boxes || nbox < 1 || !out) return NSG_ERR_INVALID;
+    *out = nullptr;
+    nsg_ensemble* e = new (std::nothrow) nsg_ensemble();
+    if (!e) return NSG_ERR_OOM;
+    *out = e;  // handed back on failure too (nsg_ensemble_last_error, then destroy)
+    e->device = boxes[0].device;
+    std::vector<std::vector<int>> members;
+    e->where.resize(nbox);
+    for (int b = 0; b < nbox; ++b) {
+        if (const char* m = check_params(&boxes[b])) return efail(e, NSG_ERR_INVALID, "box " + std::to_string(b) + ": " + m);
+        if (boxes[b].device != e->device) return efail(e, NSG_ERR_INVALID, "all boxes of an ensemble must use one device");
+        int gi = -1;
+        for (size_t g = 0; g < members.size(); ++g)
+            if (same_batch(boxes[members[g][0]], boxes[b])) { gi = (int)g; break; }
+        if (gi < 0) { members.emplace_back(); gi = (int)members.size() - 1; }
+        e->where[b] = {gi, (int)members[gi].size()};
+        members[gi].push_back(b);
+    }
+    EHIP(e, hipSetDevice(e->device));
+    for (auto& m : members) {
+        std::vector<nsg_params> ps;
+        for (int b : m) ps.push_back(boxes[b]);
+        nsg_ctx* g = nullptr;
+        const int rc = create_batch(ps.data(), (int)ps.size(), &g);
+        if (g) e->groups.push_back(g);
+        if (rc) return egroup(e, g, rc);
+    }
+    EHIP(e, hipStreamCreateWithFlags(&e->master, hipStreamNonBlocking));
+    EHIP(e, hipEventCreateWithFlags(&e->fork, hipEventDisableTiming));
+    EHIP(e, hipEventCreate(&e->ev[0]));
+    EHIP(e, hipEventCreate(&e->ev[1]));
+    e->join.assign(e->groups.size(), nullptr);
+    for (auto& ev : e->join) EHIP(e, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    return NSG_OK;
+}
+
+static void ens_drop_graph(nsg_ensemble* e) {
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->exec = nullptr;
+    e->graph = nullptr;
+}
+
+static int ens_build_graph(nsg_ensemble* e) {
+    if (e->exec) return NSG_OK;
+    EHIP(e, hipStreamBeginCapture(e->master, hipStreamCaptureModeThreadLocal));
+    int rc = NSG_OK;
+    std::string msg;
+    hipError_t st = hipEventRecord(e->fork, e->master);
+    if (st != hipSuccess) { rc = NSG_ERR_DEVICE; msg = std::string("hipEventRecord: ") + hipGetErrorString(st); }
+    for (size_t g = 0; g < e->groups.size() && rc == NSG_OK; ++g) {
+        nsg_ctx* c = e->groups[g];
+        if ((st = hipStreamWaitEvent(c->s, e->fork, 0)) != hipSuccess) {
+            rc = NSG_ERR_DEVICE; msg = std::string("hipStreamWaitEvent: ") + hipGetErrorString(st); break;
+        }
+        if ((rc = enqueue_steps(c, 1, 0, c->nbox))) { msg = c->err; break; }
+        if ((st = hipEventRecord(e->join[g], c->s)) != hipSuccess ||
+            (st = hipStreamWaitEvent(e->master, e->join[g], 0)) != hipSuccess) {
+            rc = NSG_ERR_DEVICE; msg = std::string("join: ") + hipGetErrorString(st); break;
+        }
+    }
+    hipGraph_t graph = nullptr;
+    st = hipStreamEndCapture(e->master, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return efail(e, rc, msg);
+    }
+    if (st != hipSuccess) return efail(e, NSG_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(st));
+    e->graph = graph;
+    EHIP(e, hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+    return NSG_OK;
+}
+
+static int ens_check(nsg_ensemble* e) {
+    for (nsg_ctx* g : e->groups) {
+        const int rc = check_boxes(g);
+        if (rc) return egroup(e, g, rc);
+    }
+    return NSG_OK;
+}
+
+int nsg_ensemble_step(nsg_ensemble* e, long nsteps) {
+    if (!e || nsteps < 0) return NSG_ERR_INVALID;
+    EHIP(e, hipSetDevice(e->device));
+    const int rc = ens_build_graph(e);
+    if (rc) return rc;
+    for (long i = 0; i < nsteps; ++i) EHIP(e, hipGraphLaunch(e->exec, e->master));
+    EHIP(e, hipStreamSynchronize(e->master));
+    return ens_check(e);
+}
+
+int nsg_ensemble_step_serial(nsg_ensemble* e, long nsteps) {
+    if (!e || nsteps < 0) return NSG_ERR_INVALID;
+    EHIP(e, hipSetDevice(e->device));
+    for (long i = 0; i < nsteps; ++i)
+        for (nsg_ctx* g : e->groups)
+            for (int b = 0; b < g->nbox; ++b) {
+                const int rc = enqueue_steps(g, 1, b, 1);
+                if (rc) return egroup(e, g, rc);
+            }
+    return ens_check(e);
+}
+
+int nsg_ensemble_autotune(nsg_ensemble* e, int iters, int use_cache, int* chosen_per_group) {
+    if (!e || iters <= 0) return NSG_ERR_INVALID;
+    EHIP(e, hipSetDevice(e->device));
+    ens_drop_graph(e);
+    for (size_t g = 0; g < e->groups.size(); ++g) {
+        int ch = 0;
+        const int rc = autotune_ctx(e->groups[g], iters, use_cache, &ch, nullptr);
+        if (rc) return egroup(e, e->groups[g], rc);
+        if (chosen_per_group) chosen_per_group[g] = ch;
+    }
+    return NSG_OK;
+}
+
+int nsg_ensemble_set_schedule(nsg_ensemble* e, int group, int sched) {
+    if (!e || group < 0 || group >= (int)e->groups.size()) return NSG_ERR_INVALID;
+    const int rc = nsg_set_schedule(e->groups[group], sched);
+    if (rc) return egroup(e, e->groups[group], rc);
+    ens_drop_graph(e);
+    return NSG_OK;
+}
+
+int nsg_ensemble_group_count(const nsg_ensemble* e) { return e ? (int)e->groups.size() : 0; }
+
+int nsg_ensemble_group_of(const nsg_ensemble* e, int box, int* group, int* index) {
+    if (!e || box < 0 || box >= (int)e->where.size()) return NSG_ERR_INVALID;
+    if (group) *group = e->where[box].first;
+    if (index) *index = e->where[box].second;
+    return NSG_OK;
+}
+
+double nsg_ensemble_step_bytes(const nsg_ensemble* e) {
+    double s = 0.0;
+    if (e)
+        for (const nsg_ctx* g : e->groups) s += nsg_step_bytes(g);
+    return s;
+}
+
+#define ENS_BOX(e, box)                                                             \
+    if (!(e) || (box) < 0 || (box) >= (int)(e)->where.size()) return NSG_ERR_INVALID; \
+    EHIP((e), hipSetDevice((e)->device));                                           \
+    nsg_ctx* g = (e)->groups[(e)->where[(box)].first];                              \
+    const int gb = (e)->where[(box)].second
+
+int nsg_ensemble_init_state(nsg_ensemble* e, int box, int kind, double k0, double energy, uint64_t seed) {
+    ENS_BOX(e, box);
+    const int rc = init_state_box(g, gb, kind, k0, energy, seed);
+    return rc ? egroup(e, g, rc) : NSG_OK;
+}
+
+int nsg_ensemble_set_state(nsg_ensemble* e, int box, const double* u_hat) {
+    ENS_BOX(e, box);
+    if (!u_hat) return NSG_ERR_INVALID;
+    const int rc = set_state_box(g, gb, u_hat, g->uh);
+    return rc ? egroup(e, g, rc) : NSG_OK;
+}
+
+int nsg_ensemble_get_state(nsg_ensemble* e, int box, double* u_hat) {
+    ENS_BOX(e, box);
+    if (!u_hat) return NSG_ERR_INVALID;
+    const int rc = download_compact(g, g->uh + gb * g->sbox, false, u_hat);
+    return rc ? egroup(e, g, rc) : NSG_OK;
+}
+
+int nsg_ensemble_measures(nsg_ensemble* e, int box, nsg_totals* out, double* E_k, double* Ens_k, int n_spect, int quirk) {
+    ENS_BOX(e, box);
+    if (!out) return NSG_ERR_INVALID;
+    const int rc = measures_box(g, gb, out, E_k, Ens_k, n_spect, quirk);
+    return rc ? egroup(e, g, rc) : NSG_OK;
+}
+
+int nsg_ensemble_realspace(nsg_ensemble* e, int box, int which, double* out) {
+    ENS_BOX(e, box);
+    if (!out || (which != 0 && which != 1)) return NSG_ERR_INVALID;
+    const int rc = realspace_box(g, gb, which, out);
+    return rc ? egroup(e, g, rc) : NSG_OK;
+}
+
+int nsg_ensemble_get_time(nsg_ensemble* e, int box, double* t, double* dt, double* max_speed) {
+    ENS_BOX(e, box);
+    const int rc = get_time_box(g, gb, t, dt, max_speed);
+    return rc ? egroup(e, g, rc) : NSG_OK;
+}
+
+int nsg_ensemble_n_spect(const nsg_ensemble* e, int box) {
+    if (!e || box < 0 || box >= (int)e->where.size()) return 0;
+    return e->groups[e->where[box].first]->nspect;
+}
+
+// wall time of nsteps graph replays on the ensemble stream (HIP events), for benchmarks
+int nsg_ensemble_time_steps(nsg_ensemble* e, long nsteps, int serial, double* ms) {
+    if (!e || nsteps <= 0 || !ms) return NSG_ERR_INVALID;
+    EHIP(e, hipSetDevice(e->device));
+    int rc;
+    if (!serial && (rc = ens_build_graph(e))) return rc;
+    for (nsg_ctx* g : e->groups) EHIP(e, hipStreamSynchronize(g->s));
+    EHIP(e, hipEventRecord(e->ev[0], e->master));
+    if (serial) {
+        // the serial path runs on the groups' streams: order them after the start event and
+        // bring them back to the master stream before the end event
+        for (nsg_ctx* g : e->groups) EHIP(e, hipStreamWaitEvent(g->s, e->ev[0], 0));
+        for (long i = 0; i < nsteps; ++i)
+            for (nsg_ctx* g : e->groups)
+                for (int b = 0; b < g->nbox; ++b)
+                    if ((rc = enqueue_steps(g, 1, b, 1))) return egroup(e, g, rc);
+        for (size_t gi = 0; gi < e->groups.size(); ++gi) {
+            EHIP(e, hipEventRecord(e->join[gi], e->groups[gi]->s));
+            EHIP(e, hipStreamWaitEvent(e->master, e->join[gi], 0));
+        }
+    } else {
+        for (long i = 0; i < nsteps; ++i) EHIP(e, hipGraphLaunch(e->exec, e->master));
+    }
+    EHIP(e, hipEventRecord(e->ev[1], e->master));
+    EHIP(e, hipEventSynchronize(e->ev[1]));
+    float f = 0.f;
+    EHIP(e, hipEventElapsedTime(&f, e->ev[0], e->ev[1]));
+    *ms = f;
+    return ens_check(e);
+}
+
+const char* nsg_ensemble_last_error(const nsg_ensemble* e) { return e ? e->err.c_str() : "null ensemble"; }
+
+void nsg_ensemble_destroy(nsg_ensemble* e) {
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    if (e->master) (void)hipStreamSynchronize(e->master);
+    ens_drop_graph(e);
+    for (nsg_ctx* g : e->groups) nsg_destroy(g);
+    for (auto ev : e->join)
+        if (ev) (void)hipEventDestroy(ev);
+    if (e->fork) (void)hipEventDestroy(e->fork);
+    if (e->ev[0]) (void)hipEventDestroy(e->ev[0]);
+    if (e->ev[1]) (void)hipEventDestroy(e->ev[1]);
+    if (e->master) (void)hipStreamDestroy(e->master);
+    delete e;
 }

@@ -26,6 +26,7 @@ struct DevScalars {
     long long steps;             // completed steps
     double cfl;                  // > 0: adaptive dt
     double dx;                   // 2 pi / N
+    double nu;                   // viscosity of this box
 };
 
 // ------------------------------------------------------------------------------------------
@@ -128,6 +129,7 @@ __device__ __forceinline__ bool tile_of_block(int ntiles, int nrows, int& tile, 
 // grid (NZP/W, NK); block W*TPF; thread (t, w) = (tid / W, tid % W)
 // fields: sel 0 -> (u, w) six fields; sel 1 -> u only; sel 2 -> w only (3 fields)
 // ------------------------------------------------------------------------------------------
+// batches: box = blockIdx.z; per-box element strides sbox (state/K) and obox (A1)
 struct P1Args {
     const double2* uh;
     const void* kprev;
@@ -135,7 +137,8 @@ struct P1Args {
     const void* twN;
     const DevScalars* sc;
     double* fslots;
-    double nu, kf2;
+    double kf2;
+    long long sbox, obox;
     int visc_pow, scheme, stage, force, sel;
     int kmax, NK, NZP;
 };
@@ -149,12 +152,16 @@ __global__ void __launch_bounds__(N / EPT * W) pass1_kernel(P1Args a) {
     __shared__ double red[N / EPT * W];
     int tile, iy;
     if (!tile_of_block<PAIR>(a.NZP / W, a.NK, tile, iy)) return;
+    const int box = blockIdx.z;
+    const double2* __restrict__ uh = a.uh + box * a.sbox;
+    const cpx_t<T>* kprev = a.kprev ? (const cpx_t<T>*)a.kprev + box * a.sbox : nullptr;
+    const DevScalars& sc = a.sc[box];
     const int tid = threadIdx.x, w = tid % W, t = tid / W;
     const int kz = tile * W + w;
     const int kmax = a.kmax, NK = a.NK, NZP = a.NZP;
     const int ky = k_of_cidx(iy, kmax, NK);
     const bool colok = kz <= kmax;
-    const double dt = a.sc->dt;
+    const double dt = sc.dt, nu = sc.nu;
     const long long plane = (long long)NK * NK * NZP;
 
     C u[3][EPT];  // stage input in the bulk precision (formed in FP64)
@@ -168,14 +175,13 @@ __global__ void __launch_bounds__(N / EPT * W) pass1_kernel(P1Args a) {
         const long long k2 = (long long)kx * kx + (long long)ky * ky + (long long)kz * kz;
         double2 ud[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) ud[c] = ok ? a.uh[c * plane + idx] : make_double2(0.0, 0.0);
+        for (int c = 0; c < 3; ++c) ud[c] = ok ? uh[c * plane + idx] : make_double2(0.0, 0.0);
         if (a.stage > 1 && ok) {
             double al, be;
-            stage_in_coef(a.scheme, a.stage, visc_rate(a.nu, a.visc_pow, k2), dt, al, be);
-            const cpx_t<T>* kp = (const cpx_t<T>*)a.kprev;
+            stage_in_coef(a.scheme, a.stage, visc_rate(nu, a.visc_pow, k2), dt, al, be);
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                double2 kv = to_d2<T>(kp[c * plane + idx]);
+                double2 kv = to_d2<T>(kprev[c * plane + idx]);
                 ud[c] = make_double2(al * ud[c].x + be * kv.x, al * ud[c].y + be * kv.y);
             }
         }
@@ -189,12 +195,12 @@ __global__ void __launch_bounds__(N / EPT * W) pass1_kernel(P1Args a) {
     }
     if (a.force && tile == 0 && (double)ky * ky <= a.kf2) {
         double s = block_sum(fpart, red);
-        if (tid == 0) a.fslots[iy] = s;
+        if (tid == 0) a.fslots[box * NK + iy] = s;
     }
 
     const int f0 = (a.sel == 2) ? 3 : 0;
     const int f1 = (a.sel == 1) ? 3 : 6;
-    C* out = (C*)a.out;
+    C* out = (C*)a.out + box * a.obox;
     const C* tw = (const C*)a.twN;
     const long long oplane = (long long)N * xplane(NK, NZP);
     for (int f = f0; f < f1; ++f) {
@@ -230,10 +236,13 @@ __global__ void __launch_bounds__(N / EPT * W) pass1_kernel(P1Args a) {
 //   DIR=-1 (pass 4): rows in = all N positions, rows out = compact ky
 // in/out layouts [F][N(x)][rows][NZP]; grid (NZP/W, N, F)
 // ------------------------------------------------------------------------------------------
+// batches: blockIdx.z = box * F + field; per-box element strides ibox, obox
 struct PYArgs {
     const void* in;
     void* out;
     const void* twN;
+    long long ibox, obox;
+    int F;
     int kmax, NK, NZP;
 };
 
@@ -244,13 +253,13 @@ __global__ void __launch_bounds__(N / EPT * W) passy_kernel(PYArgs a) {
     __shared__ C lds[N * W];
     const int tid = threadIdx.x, w = tid % W, t = tid / W;
     const int kz = blockIdx.x * W + w;
-    const int x = blockIdx.y, f = blockIdx.z;
+    const int x = blockIdx.y, f = blockIdx.z % a.F, box = blockIdx.z / a.F;
     const int kmax = a.kmax, NK = a.NK, NZP = a.NZP;
     // DIR>0: in = A1, out = A2;  DIR<0: in = A3, out = A4 (A1/A4 planes padded by XPAD)
     const long long fin = (DIR > 0) ? (long long)N * xplane(NK, NZP) : (long long)N * N * NZP;
     const long long fout = (DIR > 0) ? (long long)N * N * NZP : (long long)N * xplane(NK, NZP);
-    const C* in = (const C*)a.in + (long long)f * fin;
-    C* out = (C*)a.out + (long long)f * fout;
+    const C* in = (const C*)a.in + box * a.ibox + (long long)f * fin;
+    C* out = (C*)a.out + box * a.obox + (long long)f * fout;
     C v[EPT];
 #pragma unroll
     for (int m = 0; m < EPT; ++m) {
@@ -289,6 +298,7 @@ __global__ void __launch_bounds__(N / EPT * W) passy_kernel(PYArgs a) {
 // ------------------------------------------------------------------------------------------
 enum ZMode { ZM_RHS = 0, ZM_REAL = 1, ZM_R2C = 2 };
 
+// batches: lines of box b are lids [b N^2, (b+1) N^2); per-box element strides ibox, obox
 struct PZArgs {
     const void* in;   // A2 (RHS, REAL) or real AoS doubles (R2C)
     void* out;        // A3 (RHS, R2C) or real AoS doubles (REAL)
@@ -296,6 +306,8 @@ struct PZArgs {
     const void* twN;  // length N table
     DevScalars* sc;
     double scale;
+    long long ibox, obox;
+    int nbox;
     int kmax, NZP;
     int track_max;
 };
@@ -366,9 +378,12 @@ __global__ void __launch_bounds__(N / 2 / EPTZ * LPB) passz_kernel(PZArgs a) {
     constexpr int MP = zlds<M, EPTZ>();
     __shared__ C lds[MP * LPB];
     const int tid = threadIdx.x, line = tid / TPF, t = tid % TPF;
-    long long lid = (long long)blockIdx.x * LPB + line;
-    const bool live = lid < (long long)N * N;
-    if (!live) lid = (long long)N * N - 1;  // idle lines recompute the last line but never store
+    long long gid = (long long)blockIdx.x * LPB + line;
+    const long long nl = (long long)N * N;
+    const bool live = gid < nl * a.nbox;
+    if (!live) gid = nl * a.nbox - 1;  // idle lines recompute the last line but never store
+    const int box = (int)(gid / nl);
+    const long long lid = gid - box * nl;
     const int base = line * MP;
     const int kmax = a.kmax, NZP = a.NZP;
     const C* twM = (const C*)a.twM;
@@ -376,8 +391,8 @@ __global__ void __launch_bounds__(N / 2 / EPTZ * LPB) passz_kernel(PZArgs a) {
     const long long fstride = (long long)N * N * NZP;
 
     if constexpr (MODE == ZM_RHS) {
-        const C* in = (const C*)a.in + lid * NZP;
-        C* out = (C*)a.out + lid * NZP;
+        const C* in = (const C*)a.in + box * a.ibox + lid * NZP;
+        C* out = (C*)a.out + box * a.obox + lid * NZP;
         C uu[3][EPTZ], ww[3][EPTZ];
 #pragma unroll
         for (int c = 0; c < 3; ++c) z_c2r_field<T, N, EPTZ>(uu[c], in + c * fstride, kmax, t, lds, base, twM, twN);
@@ -392,8 +407,18 @@ __global__ void __launch_bounds__(N / 2 / EPTZ * LPB) passz_kernel(PZArgs a) {
                 mx = nanmax(mx, nanmax(s0, s1));
             }
             mx *= a.scale;
-            for (int off = 32; off > 0; off >>= 1) mx = nanmax(mx, __shfl_xor(mx, off));
-            if ((tid & 63) == 0 && live) atomicMax(&a.sc->maxbits, (unsigned long long)__double_as_longlong(mx));
+            // a wavefront's lines are consecutive and N^2 >= 64, so they span at most two boxes
+            // (b0 and b0 + 1) when N^2 is not a multiple of the lines per wavefront
+            const int b0 = __shfl(box, 0);
+            double ma = (box == b0) ? mx : 0.0, mb = (box == b0) ? 0.0 : mx;
+            for (int off = 32; off > 0; off >>= 1) {
+                ma = nanmax(ma, __shfl_xor(ma, off));
+                mb = nanmax(mb, __shfl_xor(mb, off));
+            }
+            if ((tid & 63) == 0) {
+                atomicMax(&a.sc[b0].maxbits, (unsigned long long)__double_as_longlong(ma));
+                if (b0 + 1 < a.nbox) atomicMax(&a.sc[b0 + 1].maxbits, (unsigned long long)__double_as_longlong(mb));
+            }
         }
         C pr[3][EPTZ];
 #pragma unroll
@@ -406,8 +431,8 @@ __global__ void __launch_bounds__(N / 2 / EPTZ * LPB) passz_kernel(PZArgs a) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) z_r2c_store<T, N, EPTZ>(pr[c], out + c * fstride, kmax, NZP, t, lds, base, twM, twN, live);
     } else if constexpr (MODE == ZM_REAL) {
-        const C* in = (const C*)a.in + lid * NZP;
-        double* out = (double*)a.out + lid * (long long)N * 3;
+        const C* in = (const C*)a.in + box * a.ibox + lid * NZP;
+        double* out = (double*)a.out + box * a.obox + lid * (long long)N * 3;
         for (int c = 0; c < 3; ++c) {
             C v[EPTZ];
             z_c2r_field<T, N, EPTZ>(v, in + c * fstride, kmax, t, lds, base, twM, twN);
@@ -421,8 +446,8 @@ __global__ void __launch_bounds__(N / 2 / EPTZ * LPB) passz_kernel(PZArgs a) {
             }
         }
     } else {  // ZM_R2C
-        const double* in = (const double*)a.in + lid * (long long)N * 3;
-        C* out = (C*)a.out + lid * NZP;
+        const double* in = (const double*)a.in + box * a.ibox + lid * (long long)N * 3;
+        C* out = (C*)a.out + box * a.obox + lid * NZP;
         for (int c = 0; c < 3; ++c) {
             C v[EPTZ];
 #pragma unroll
@@ -528,25 +553,45 @@ __global__ void __launch_bounds__(64 * WPB, OCC) passz_rhs_kernel(PZArgs a) {
     for (int m = 0; m < EPTZ; ++m) ek[m] = ((const C*)a.twN)[t + TPF * m];
     const int base = (wv * LPW + sub) * MP;
     const int kmax = a.kmax, NZP = a.NZP;
-    const long long nlines = (long long)N * N;
-    const long long ngroups = (nlines + LPW - 1) / LPW;
+    const long long nlines = (long long)N * N;        // lines per box (a multiple of LPW)
+    const long long total = nlines * a.nbox;
+    const long long ngroups = (total + LPW - 1) / LPW;
     const long long fstride = nlines * NZP;
     const C* in0 = (const C*)a.in;
     C* out0 = (C*)a.out;
+    // line address of global line id gl (box-major)
+    auto line_off = [&](long long gl, long long stride) {
+        const long long b = gl / nlines;
+        return b * stride + (gl - b * nlines) * NZP;
+    };
     double mx = 0.0;
+    int mx_box = -1;  // box whose maximum mx holds (uniform across the wavefront)
+    auto flush_max = [&]() {
+        if (a.track_max && mx_box >= 0) {
+            double v = mx * a.scale;
+            for (int off = 32; off > 0; off >>= 1) v = nanmax(v, __shfl_xor(v, off));
+            if (lane == 0) atomicMax(&a.sc[mx_box].maxbits, (unsigned long long)__double_as_longlong(v));
+        }
+        mx = 0.0;
+    };
 
     long long g = (long long)blockIdx.x * WPB + wv;
     C cur[EPTZ];
     if (g < ngroups) {
-        long long lid = g * LPW + sub;
-        if (lid >= nlines) lid = nlines - 1;
-        zr_load<T, N, EPTZ>(cur, in0 + lid * NZP, kmax, t);
+        long long gl = g * LPW + sub;
+        if (gl >= total) gl = total - 1;
+        zr_load<T, N, EPTZ>(cur, in0 + line_off(gl, a.ibox), kmax, t);
     }
     for (; g < ngroups; g += (long long)gridDim.x * WPB) {
-        long long lid = g * LPW + sub;
-        const bool live = lid < nlines;
-        if (!live) lid = nlines - 1;  // idle lanes recompute the last line but never store
-        const C* in = in0 + lid * NZP;
+        long long gl = g * LPW + sub;
+        const bool live = gl < total;
+        if (!live) gl = total - 1;  // idle lanes recompute the last line but never store
+        const int box = (int)((g * LPW) / nlines);
+        if (box != mx_box) {
+            flush_max();
+            mx_box = box;
+        }
+        const C* in = in0 + line_off(gl, a.ibox);
         C ur[3][EPTZ], pr[3][EPTZ];
 #pragma unroll
         for (int f = 0; f < 6; ++f) {
@@ -579,8 +624,8 @@ __global__ void __launch_bounds__(64 * WPB, OCC) passz_rhs_kernel(PZArgs a) {
         const long long gn = g + (long long)gridDim.x * WPB;
         if (gn < ngroups) {
             long long ln = gn * LPW + sub;
-            if (ln >= nlines) ln = nlines - 1;
-            zr_load<T, N, EPTZ>(cur, in0 + ln * NZP, kmax, t);
+            if (ln >= total) ln = total - 1;
+            zr_load<T, N, EPTZ>(cur, in0 + line_off(ln, a.ibox), kmax, t);
         }
         if (a.track_max && live) {
 #pragma unroll
@@ -590,15 +635,11 @@ __global__ void __launch_bounds__(64 * WPB, OCC) passz_rhs_kernel(PZArgs a) {
                 mx = nanmax(mx, nanmax(s0, s1));
             }
         }
-        C* out = out0 + lid * NZP;
+        C* out = out0 + line_off(gl, a.obox);
 #pragma unroll
         for (int c = 0; c < 3; ++c) zr_r2c_store<T, N, EPTZ, SY>(pr[c], out + c * fstride, kmax, NZP, t, lds, base, wz, ek, live);
     }
-    if (a.track_max) {
-        mx *= a.scale;
-        for (int off = 32; off > 0; off >>= 1) mx = nanmax(mx, __shfl_xor(mx, off));
-        if (lane == 0) atomicMax(&a.sc->maxbits, (unsigned long long)__double_as_longlong(mx));
-    }
+    flush_max();
 }
 
 // ------------------------------------------------------------------------------------------
@@ -610,6 +651,7 @@ __global__ void __launch_bounds__(64 * WPB, OCC) passz_rhs_kernel(PZArgs a) {
 // ------------------------------------------------------------------------------------------
 enum EpMode { EP_STAGE = 0, EP_STORE = 1 };
 
+// batches: box = blockIdx.z; per-box element strides sbox (state/K) and ibox (A4)
 struct P5Args {
     const void* in;   // A4
     double2* uh;      // state
@@ -620,7 +662,8 @@ struct P5Args {
     const void* twN;
     const DevScalars* sc;
     const double* fslots;
-    double nu, kf2, eps_in, norm;
+    double kf2, eps_in, norm;
+    long long sbox, ibox;
     int visc_pow, scheme, stage, force;
     int kmax, NK, NZP;
 };
@@ -633,13 +676,21 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
     __shared__ C lds[N * W];
     int tile, iy;
     if (!tile_of_block<PAIR>(a.NZP / W, a.NK, tile, iy)) return;
+    const int box = blockIdx.z;
+    const DevScalars& sc = a.sc[box];
+    double2* __restrict__ uh = a.uh + box * a.sbox;
+    const C* K1 = (const C*)a.k1 + box * a.sbox;
+    const C* K2 = (const C*)a.k2 + box * a.sbox;
+    const C* K3 = (const C*)a.k3 + box * a.sbox;
+    C* ko = a.kout ? (C*)a.kout + box * a.sbox : nullptr;
+    const double* fslots = a.fslots + box * a.NK;
     const int tid = threadIdx.x, w = tid % W, t = tid / W;
     const int kz = tile * W + w;
     const int kmax = a.kmax, NK = a.NK, NZP = a.NZP;
     const int ky = k_of_cidx(iy, kmax, NK);
     const long long plane = (long long)NK * NK * NZP;
     const long long iplane = (long long)N * xplane(NK, NZP);
-    const C* in = (const C*)a.in;
+    const C* in = (const C*)a.in + box * a.ibox;
     const C* tw = (const C*)a.twN;
 
     C r[3][EPT];
@@ -656,12 +707,12 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
         double s = 0.0;
         int kfi = (int)floor(sqrt(a.kf2));
         if (kfi > kmax) kfi = kmax;
-        for (int q = -kfi; q <= kfi; ++q) s += a.fslots[cidx_of_k(q, NK)];
+        for (int q = -kfi; q <= kfi; ++q) s += fslots[cidx_of_k(q, NK)];
         const double N3 = (double)N * N * N;
         const double Ef = 0.5 * s / (N3 * N3);
         ffac = (Ef > 0.0) ? a.eps_in / (2.0 * Ef) : 0.0;
     }
-    const double dt = a.sc->dt;
+    const double dt = sc.dt, nu = sc.nu;
     const long long kcut2 = (long long)kmax * kmax;
 
 #pragma unroll
@@ -678,7 +729,7 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
             for (int c = 0; c < 3; ++c)
                 nl[c] = keep ? make_double2((double)r[c][m].x * a.norm, (double)r[c][m].y * a.norm) : make_double2(0.0, 0.0);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) a.uh[c * plane + idx] = nl[c];
+            for (int c = 0; c < 3; ++c) uh[c * plane + idx] = nl[c];
             continue;
         }
 #pragma unroll
@@ -695,14 +746,14 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) nl[c] = make_double2(0.0, 0.0);
         }
-        const double D = visc_rate(a.nu, a.visc_pow, k2);
+        const double D = visc_rate(nu, a.visc_pow, k2);
         if (a.force && keep && k2 > 0 && (double)k2 <= a.kf2) {
             double al, be;
             stage_in_coef(a.scheme, a.stage, D, dt, al, be);
-            const C* kp = (const C*)(a.stage == 2 ? a.k1 : a.stage == 3 ? a.k2 : a.k3);
+            const C* kp = (a.stage == 2) ? K1 : (a.stage == 3) ? K2 : K3;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                double2 us = a.uh[c * plane + idx];
+                double2 us = uh[c * plane + idx];
                 if (a.stage > 1) {
                     double2 kv = to_d2<T>(kp[c * plane + idx]);
                     us = make_double2(al * us.x + be * kv.x, al * us.y + be * kv.y);
@@ -712,7 +763,6 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
             }
         }
         if (a.stage < 4) {
-            C* ko = (C*)a.kout;
 #pragma unroll
             for (int c = 0; c < 3; ++c) ko[c * plane + idx] = from_d2<T>(nl[c]);
         } else {
@@ -726,12 +776,9 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
                 g = exp(-D * dt);
                 e2 = exp(-D * dt * 0.5);
             }
-            const C* K1 = (const C*)a.k1;
-            const C* K2 = (const C*)a.k2;
-            const C* K3 = (const C*)a.k3;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const double2 u0 = a.uh[c * plane + idx];
+                const double2 u0 = uh[c * plane + idx];
                 const double2 q1 = to_d2<T>(K1[c * plane + idx]);
                 const double2 q2 = to_d2<T>(K2[c * plane + idx]);
                 const double2 q3 = to_d2<T>(K3[c * plane + idx]);
@@ -746,7 +793,7 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
                     un.y = g * u0.y + dt / 6.0 * (g * q1.y + 2.0 * e2 * (q2.y + q3.y) + nl[c].y);
                 }
                 if (!keep) un = make_double2(0.0, 0.0);
-                a.uh[c * plane + idx] = un;
+                uh[c * plane + idx] = un;
             }
         }
     }
@@ -755,20 +802,20 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
 // ------------------------------------------------------------------------------------------
 // Step bookkeeping (single-thread kernels; no memsets so the step can be graph-captured)
 // ------------------------------------------------------------------------------------------
-__global__ void step_begin_kernel(DevScalars* sc) {
-    if (threadIdx.x == 0) sc->maxbits = 0ull;
+__global__ void step_begin_kernel(DevScalars* sc, int nbox) {
+    for (int b = threadIdx.x; b < nbox; b += blockDim.x) sc[b].maxbits = 0ull;
 }
-__global__ void cfl_kernel(DevScalars* sc) {
-    if (threadIdx.x == 0) {
-        const double mx = __longlong_as_double((long long)sc->maxbits);
-        if (sc->cfl > 0.0 && mx > 0.0) sc->dt = sc->cfl * sc->dx / mx;
+__global__ void cfl_kernel(DevScalars* sc, int nbox) {
+    for (int b = threadIdx.x; b < nbox; b += blockDim.x) {
+        const double mx = __longlong_as_double((long long)sc[b].maxbits);
+        if (sc[b].cfl > 0.0 && mx > 0.0) sc[b].dt = sc[b].cfl * sc[b].dx / mx;
     }
 }
-__global__ void step_end_kernel(DevScalars* sc) {
-    if (threadIdx.x == 0) {
-        sc->t += sc->dt;
-        sc->steps += 1;
-        sc->max_last = __longlong_as_double((long long)sc->maxbits);
+__global__ void step_end_kernel(DevScalars* sc, int nbox) {
+    for (int b = threadIdx.x; b < nbox; b += blockDim.x) {
+        sc[b].t += sc[b].dt;
+        sc[b].steps += 1;
+        sc[b].max_last = __longlong_as_double((long long)sc[b].maxbits);
     }
 }
 
@@ -827,10 +874,14 @@ __device__ __forceinline__ long long isqrt_ll(long long v) {
     return s;
 }
 
-__global__ void measures_kernel(const double2* __restrict__ uh, double* __restrict__ part, int N, int kmax, int NK,
-                                int NZP, double nu, int visc_pow, int quirk) {
+// grid (n_spect, nbox): part[(box * n_spect + b) * 7 + j]
+__global__ void measures_kernel(const double2* __restrict__ uh0, double* __restrict__ part0, int N, int kmax, int NK,
+                                int NZP, const DevScalars* __restrict__ sc, int visc_pow, int quirk, long long sbox) {
     __shared__ double red[256];
-    const int b = blockIdx.x;
+    const int b = blockIdx.x, box = blockIdx.y;
+    const double2* __restrict__ uh = uh0 + box * sbox;
+    double* __restrict__ part = part0 + (long long)box * gridDim.x * 7;
+    const double nu = sc[box].nu;
     const long long lo = (b == 0) ? 0 : (long long)b * b - b + 1;
     const long long hi = (b == 0) ? 0 : (long long)b * b + b;
     const long long plane = (long long)NK * NK * NZP;

@@ -11,6 +11,10 @@ and reported under "fp64".  `roofline` describes the kernel with the largest sha
 timed with HIP events on the library's own stream.  `cpu_baseline` times the FP64 CPU oracle
 (a C restatement of the reference algorithm, OpenMP) on a bounded sample: one nonlinear-term
 evaluation at 512^3, scaled to steps/s as 1/(4 x that time).
+
+"ensemble_96_boxes" reports the batched-ensemble workload: 96 independent FP32-bulk boxes (32 at
+32^3, 32 at 48^3, 24 at 64^3, 8 at 96^3), 100 RK4 steps, autotuned FFT schedules, one HIP graph
+replay per ensemble step, against the same steps launched box by box.
 """
 from __future__ import annotations
 
@@ -89,6 +93,54 @@ def run_mode(nsg, N, precision, steps, warmup, prof_iters):
     return s, out
 
 
+ENSEMBLE_SIZES = [32] * 32 + [48] * 32 + [64] * 24 + [96] * 8
+
+
+def ensemble_boxes():
+    """96 uneven boxes: 32 at 32^3, 32 at 48^3, 24 at 64^3, 8 at 96^3; box b: nu = 1e-3 + 4e-3 b/95,
+    FP32-bulk, RK4, CFL 0.5 (k^4 start: seed 1000+b, k0 = 2 + b mod 3)"""
+    return [dict(N=N, nu=1e-3 + 4e-3 * b / 95, dt=1e-3, precision="fp32bulk", cfl=0.5)
+            for b, N in enumerate(ENSEMBLE_SIZES)]
+
+
+def run_ensemble(nsg, steps, warmup):
+    """aggregate box-steps/s of the 96-box ensemble: autotuned schedules, one HIP graph per step, vs
+    the same steps launched box by box with direct launches (same schedules)"""
+    boxes = ensemble_boxes()
+
+    def fresh(e):
+        for b in range(len(boxes)):
+            e.init(b, "k4", k0=2.0 + b % 3, energy=0.5, seed=1000 + b)
+
+    with nsg.Ensemble(boxes) as e:
+        fresh(e)
+        log("[bench] ensemble: autotune")
+        chosen = e.autotune(iters=3, use_cache=False)
+        fresh(e)
+        e.step(warmup)
+        ms_graph = e.time_steps(steps, serial=False)
+        gbytes = e.step_bytes()
+    with nsg.Ensemble(boxes) as r:
+        for g, c in enumerate(chosen):
+            r.set_schedule(g, c)
+        fresh(r)
+        r.step_serial(warmup)
+        ms_serial = r.time_steps(steps, serial=True)
+    nb = len(boxes)
+    out = {"boxes": nb, "sizes": {"32": 32, "48": 32, "64": 24, "96": 8}, "steps": steps,
+           "schedules": dict(zip(["32", "48", "64", "96"], chosen)),
+           "graph_ms_per_ensemble_step": ms_graph / steps,
+           "graph_box_steps_per_s": nb * steps / (ms_graph * 1e-3),
+           "serial_ms_per_ensemble_step": ms_serial / steps,
+           "serial_box_steps_per_s": nb * steps / (ms_serial * 1e-3),
+           "speedup_vs_serial": ms_serial / ms_graph,
+           "algorithmic_GB_per_ensemble_step": gbytes / 1e9,
+           "graph_GBps": gbytes / (ms_graph * 1e-3 / steps) / 1e9}
+    log(f"[bench] ensemble: graph {out['graph_box_steps_per_s']:.0f} box-steps/s, serial "
+        f"{out['serial_box_steps_per_s']:.0f}, x{out['speedup_vs_serial']:.2f}")
+    return out
+
+
 def cpu_baseline(N):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O  # the checker; timed here only as the reported CPU baseline
@@ -113,6 +165,7 @@ def main():
     ap.add_argument("--prof-iters", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=512)
+    ap.add_argument("--ensemble-steps", type=int, default=100, help="0 skips the 96-box ensemble")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -154,6 +207,8 @@ def main():
     s64.close()
     s32.close()
 
+    ens = run_ensemble(nsg, args.ensemble_steps, 3) if args.ensemble_steps > 0 else None
+
     cpu = None
     if not args.no_cpu_baseline:
         log(f"[bench] cpu baseline at {args.cpu_n}^3 ...")
@@ -188,6 +243,7 @@ def main():
         "passes": r32["passes"],
         "step_bytes_GB": r32["step_bytes_GB"],
         "step_GBps": r32["step_GBps"],
+        "ensemble_96_boxes": ens,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

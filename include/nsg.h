@@ -102,6 +102,47 @@ int nsg_profile_passes(nsg_ctx* ctx, int iters, double* ms, double* bytes);
 /* algorithmic HBM bytes of one full RK4 step (sum over the 20 pass launches) */
 double nsg_step_bytes(const nsg_ctx* ctx);
 
+/* FFT launch schedules.  Every size has schedule 0; N = 32, 48, 64, 96 have alternatives
+ * (1: shorter radices, more threads per transform; 2: wider column tiles).  Schedules differ only
+ * in rounding.  nsg_autotune times `iters` steps of each on the context's own state, restores the
+ * state and time afterwards, keeps the fastest and caches the choice per (N, precision) for the
+ * process (use_cache = 0 re-measures).  ms_per_step, if not NULL, receives one entry per schedule.
+ * No reference counterpart: the reference is planned by FFTW_MEASURE (Solver/src/solver.c:2034-2073). */
+int nsg_schedule_count(int N);
+int nsg_set_schedule(nsg_ctx* ctx, int sched);
+int nsg_autotune(nsg_ctx* ctx, int iters, int use_cache, int* chosen, double* ms_per_step);
+
+/* Ensembles: many independent boxes on one GPU.  The reference runs one box per process and
+ * sweeps parameters with one solver invocation per run (RunScripts/run_stats*.sh call the
+ * binaries once per case); an ensemble replaces that loop of SpectralSolve runs
+ * (Solver/src/solver.c:50-207).  Boxes sharing N, precision, scheme, visc_pow, forcing and device
+ * form one batch (one launch per pass covers the whole batch); nu, dt, t0, cfl and max_speed are
+ * per box.  nsg_ensemble_step replays a HIP graph holding one step of every batch, the batches
+ * on their own streams; each box evolves exactly as it would alone under the same schedule.
+ * Box indices follow the order of `boxes`. */
+typedef struct nsg_ensemble nsg_ensemble;
+int nsg_ensemble_create(const nsg_params* boxes, int nbox, nsg_ensemble** out);
+void nsg_ensemble_destroy(nsg_ensemble* e);
+const char* nsg_ensemble_last_error(const nsg_ensemble* e);
+int nsg_ensemble_group_count(const nsg_ensemble* e);
+int nsg_ensemble_group_of(const nsg_ensemble* e, int box, int* group, int* index);
+int nsg_ensemble_n_spect(const nsg_ensemble* e, int box);
+int nsg_ensemble_init_state(nsg_ensemble* e, int box, int kind, double k0, double energy, uint64_t seed);
+int nsg_ensemble_set_state(nsg_ensemble* e, int box, const double* u_hat);
+int nsg_ensemble_get_state(nsg_ensemble* e, int box, double* u_hat);
+int nsg_ensemble_measures(nsg_ensemble* e, int box, nsg_totals* out, double* E_k, double* Ens_k, int n_spect, int quirk);
+int nsg_ensemble_realspace(nsg_ensemble* e, int box, int which, double* out);
+int nsg_ensemble_get_time(nsg_ensemble* e, int box, double* t, double* dt, double* max_speed);
+/* nsteps steps of every box: graph replays (nsg_ensemble_step) or one box at a time with direct
+ * launches (nsg_ensemble_step_serial, the serial per-box reference) */
+int nsg_ensemble_step(nsg_ensemble* e, long nsteps);
+int nsg_ensemble_step_serial(nsg_ensemble* e, long nsteps);
+/* nsteps steps timed with HIP events on the ensemble's stream (serial = 0: graph; 1: per box) */
+int nsg_ensemble_time_steps(nsg_ensemble* e, long nsteps, int serial, double* ms);
+int nsg_ensemble_autotune(nsg_ensemble* e, int iters, int use_cache, int* chosen_per_group);
+int nsg_ensemble_set_schedule(nsg_ensemble* e, int group, int sched);
+double nsg_ensemble_step_bytes(const nsg_ensemble* e);
+
 #ifdef __cplusplus
 }
 #endif
