@@ -109,6 +109,30 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.nsg_ensemble_set_schedule.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     lib.nsg_ensemble_step_bytes.argtypes = [vp]
     lib.nsg_ensemble_step_bytes.restype = ctypes.c_double
+    # learned closure
+    fp = P(ctypes.c_float)
+    lib.nsg_cnn_param_count.restype = ctypes.c_longlong
+    lib.nsg_cnn_param_count.argtypes = []
+    lib.nsg_cnn_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, P(vp)]
+    lib.nsg_cnn_destroy.argtypes = [vp]
+    lib.nsg_cnn_last_error.argtypes = [vp]
+    lib.nsg_cnn_last_error.restype = ctypes.c_char_p
+    lib.nsg_cnn_set_params.argtypes = [vp, vp]
+    lib.nsg_cnn_get_params.argtypes = [vp, vp]
+    lib.nsg_cnn_set_norm.argtypes = [vp, ctypes.c_double, dp]
+    lib.nsg_cnn_get_norm.argtypes = [vp, dp, dp]
+    lib.nsg_cnn_forward.argtypes = [vp, vp, ctypes.c_int, vp]
+    lib.nsg_cnn_loss_grad.argtypes = [vp, vp, vp, ctypes.c_int, dp, vp]
+    lib.nsg_cnn_add_sample.argtypes = [vp, vp, vp]
+    lib.nsg_cnn_sample_count.argtypes = [vp]
+    lib.nsg_cnn_train.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                  ctypes.c_uint64, dp]
+    lib.nsg_cnn_eval.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
+    lib.nsg_cnn_time_forward.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
+    lib.nsg_sgs_sample.argtypes = [vp, vp, ctypes.c_double, vp, vp]
+    lib.nsg_attach_closure.argtypes = [vp, vp]
+    lib.nsg_resample.argtypes = [vp, vp]
+    lib.nsg_closure_term.argtypes = [vp, vp]
     _lib = lib
     return lib
 
@@ -258,6 +282,35 @@ class Solver:
     def set_schedule(self, sched: int):
         self._check(self.lib.nsg_set_schedule(self._h, int(sched)))
 
+    # -- learned closure -------------------------------------------------------------------
+    def sgs_sample(self, les: "Solver", kc: float, want_x: bool = True, want_tau: bool = True):
+        """Filter this (DNS) box into `les` and return (x [Nl^3, 9], tau [Nl^3, 6]) as float32:
+        the velocity gradients of the filtered field and the deviatoric subgrid stress."""
+        n = les.N ** 3
+        x = np.empty((n, 9), dtype=np.float32) if want_x else None
+        t = np.empty((n, 6), dtype=np.float32) if want_tau else None
+        rc = self.lib.nsg_sgs_sample(self._h, les._h, float(kc), _ptr(x) if want_x else None,
+                                     _ptr(t) if want_tau else None)
+        if rc != 0:
+            raise NSGError(rc, self.lib.nsg_last_error(les._h).decode() or self.lib.nsg_last_error(self._h).decode())
+        return x, t
+
+    def resample_to(self, dst: "Solver"):
+        """copy this box's state onto dst's grid (spectral truncation or zero-padding) and time"""
+        rc = self.lib.nsg_resample(self._h, dst._h)
+        if rc != 0:
+            raise NSGError(rc, self.lib.nsg_last_error(dst._h).decode())
+
+    def attach_closure(self, net):
+        """Add the network's subgrid term to every later step (None detaches)."""
+        self._check(self.lib.nsg_attach_closure(self._h, net._h if net is not None else None))
+        self._net = net  # the network must outlive the attachment
+
+    def closure_term(self) -> np.ndarray:
+        out = np.empty(self.spectral_shape, dtype=np.complex128)
+        self._check(self.lib.nsg_closure_term(self._h, _ptr(out)))
+        return out
+
     def autotune(self, iters: int = 3, use_cache: bool = True):
         """Time every FFT schedule of this size on the current state (restored afterwards) and keep
         the fastest; returns (chosen schedule, ms per step of each schedule or None if cached)."""
@@ -386,3 +439,122 @@ class Ensemble:
 
     def step_bytes(self) -> float:
         return self.lib.nsg_ensemble_step_bytes(self._h)
+
+
+class Network:
+    """The learned subgrid-stress network on one GPU (a `nsg_cnn`): periodic 3x3x3 convolutions
+    9 -> 32 -> 32 -> 32 -> 6 with GELU, BF16 MFMA kernels, on an N^3 grid."""
+
+    def __init__(self, N: int, max_batch: int = 1, seed: int = 1, device: int = 0):
+        self.lib = load_library()
+        h = ctypes.c_void_p()
+        rc = self.lib.nsg_cnn_create(int(N), int(max_batch), int(seed), int(device), ctypes.byref(h))
+        self._h = h
+        if rc != 0:
+            msg = self.lib.nsg_cnn_last_error(h).decode() if h.value else "nsg_cnn_create failed"
+            if h.value:
+                self.lib.nsg_cnn_destroy(h)
+                self._h = ctypes.c_void_p()
+            raise NSGError(rc, msg)
+        self.N, self.max_batch = int(N), int(max_batch)
+        self.param_count = int(self.lib.nsg_cnn_param_count())
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise NSGError(rc, self.lib.nsg_cnn_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self.lib.nsg_cnn_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def get_params(self) -> np.ndarray:
+        out = np.empty(self.param_count, dtype=np.float32)
+        self._check(self.lib.nsg_cnn_get_params(self._h, _ptr(out)))
+        return out
+
+    def set_params(self, p: np.ndarray):
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        if a.size != self.param_count:
+            raise ValueError(f"expected {self.param_count} parameters")
+        self._check(self.lib.nsg_cnn_set_params(self._h, _ptr(a)))
+
+    def set_norm(self, sigma_in: float, sigma_out):
+        so = (ctypes.c_double * 6)(*[float(v) for v in sigma_out])
+        self._check(self.lib.nsg_cnn_set_norm(self._h, float(sigma_in), so))
+
+    @property
+    def norm(self):
+        si = ctypes.c_double()
+        so = (ctypes.c_double * 6)()
+        self._check(self.lib.nsg_cnn_get_norm(self._h, ctypes.byref(si), so))
+        return si.value, list(so)
+
+    def _fields(self, a, C):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        B = a.size // (self.N ** 3 * C)
+        if a.size != B * self.N ** 3 * C or not 1 <= B <= self.max_batch:
+            raise ValueError(f"expected [B, N^3, {C}] with 1 <= B <= {self.max_batch}")
+        return a, B
+
+    def forward(self, x: np.ndarray) -> np.ndarray:
+        """normalised input [B, N^3, 9] (or [B, N, N, N, 9]) -> normalised output [B, N^3, 6]"""
+        a, B = self._fields(x, 9)
+        out = np.empty((B, self.N ** 3, 6), dtype=np.float32)
+        self._check(self.lib.nsg_cnn_forward(self._h, _ptr(a), B, _ptr(out)))
+        return out
+
+    def loss_grad(self, x: np.ndarray, t: np.ndarray, grads: bool = True):
+        a, B = self._fields(x, 9)
+        tt, B2 = self._fields(t, 6)
+        if B2 != B:
+            raise ValueError("x and t batch sizes differ")
+        loss = ctypes.c_double()
+        g = np.empty(self.param_count, dtype=np.float32) if grads else None
+        self._check(self.lib.nsg_cnn_loss_grad(self._h, _ptr(a), _ptr(tt), B, ctypes.byref(loss),
+                                               _ptr(g) if grads else None))
+        return loss.value, g
+
+    def add_sample(self, x: np.ndarray, tau: np.ndarray):
+        """physical velocity gradients [N^3, 9] and deviatoric stress [N^3, 6]"""
+        a, B = self._fields(x, 9)
+        t, B2 = self._fields(tau, 6)
+        if B != 1 or B2 != 1:
+            raise ValueError("one sample at a time")
+        self._check(self.lib.nsg_cnn_add_sample(self._h, _ptr(a), _ptr(t)))
+
+    @property
+    def samples(self) -> int:
+        return self.lib.nsg_cnn_sample_count(self._h)
+
+    def train(self, steps: int, batch: int = 1, lr: float = 1e-3, seed: int = 1, first: int = 0,
+              count: int = None) -> np.ndarray:
+        """Adam steps on random batches from samples [first, first + count); returns the losses"""
+        count = self.samples - first if count is None else count
+        hist = np.zeros(max(int(steps), 1))
+        self._check(self.lib.nsg_cnn_train(self._h, int(first), int(count), int(steps), int(batch), float(lr),
+                                           int(seed), hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return hist[:int(steps)]
+
+    def eval(self, first: int = 0, count: int = None) -> float:
+        count = self.samples - first if count is None else count
+        loss = ctypes.c_double()
+        self._check(self.lib.nsg_cnn_eval(self._h, int(first), int(count), ctypes.byref(loss)))
+        return loss.value
+
+    def time_forward(self, batch: int = 1, iters: int = 10) -> float:
+        ms = ctypes.c_double()
+        self._check(self.lib.nsg_cnn_time_forward(self._h, int(batch), int(iters), ctypes.byref(ms)))
+        return ms.value

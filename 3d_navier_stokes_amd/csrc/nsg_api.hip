@@ -23,6 +23,7 @@
 
 #include "../../include/nsg.h"
 #include "nsg_kernels.hpp"
+#include "nsg_internal.hpp"
 
 using namespace nsg;
 
@@ -98,63 +99,6 @@ template <typename F> static int dispatch_NS(int N, int S, F&& f) {
         return f(n, std::integral_constant<int, 0>{});
     });
 }
-
-struct nsg_ctx {
-    nsg_params p;  // parameters of box 0; the ones a batch shares apply to every box
-    int N = 0, kmax = 0, NK = 0, NZP = 0, nspect = 0, nbox = 1;
-    int sched = 0;
-    bool fp32 = false, any_cfl = false;
-    int ncu = 256, zblocks_per_cu = 8;  // persistent z-pass grid = ncu * zblocks_per_cu blocks
-    int zocc = 1;                       // >= 3: z kernel compiled for 3 waves/SIMD (fewer VGPRs)
-    hipStream_t s = nullptr;
-    // per-box element strides of the state/K arrays and of A1..A4
-    long long sbox = 0, a1box = 0, a2box = 0, a3box = 0, a4box = 0;
-    double2* uh = nullptr;   // states [nbox][3][NK][NK][NZP]
-    double2* aux = nullptr;  // a second FP64 state per box (nsg_rhs input, autotuner backup)
-    void* K[3] = {nullptr, nullptr, nullptr};
-    void* B1 = nullptr;  // A1 / A3 of every box, or one box's real AoS field
-    void* B2 = nullptr;  // A2 / A4 of every box, or one box's AoS staging
-    double2 *twN64 = nullptr, *twM64 = nullptr;
-    float2 *twN32 = nullptr, *twM32 = nullptr;
-    DevScalars* sc = nullptr;  // [nbox]
-    double* fslots = nullptr;  // [nbox][NK]
-    double* part = nullptr;
-    size_t part_n = 0;
-    std::vector<double> max_speed;  // per-box blow-up limit (0 = none)
-    std::string err;
-    // profiling
-    bool prof = false;
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    double prof_ms[5] = {0, 0, 0, 0, 0};
-    long prof_cnt[5] = {0, 0, 0, 0, 0};
-};
-
-static int fail(nsg_ctx* c, int code, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
-static int fail(nsg_ctx* c, int code, const char* fmt, ...) {
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
-    va_end(ap);
-    if (c) c->err = buf;
-    return code;
-}
-
-#define HIPCHK(ctx, expr)                                                                        \
-    do {                                                                                         \
-        hipError_t e_ = (expr);                                                                  \
-        if (e_ != hipSuccess)                                                                    \
-            return fail((ctx), NSG_ERR_DEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
-                        __FILE__, __LINE__);                                                     \
-    } while (0)
-
-#define LAUNCHCHK(ctx)                                                                            \
-    do {                                                                                          \
-        hipError_t e_ = hipGetLastError();                                                        \
-        if (e_ != hipSuccess)                                                                     \
-            return fail((ctx), NSG_ERR_DEVICE, "kernel launch failed: %s (%s:%d)", hipGetErrorString(e_), \
-                        __FILE__, __LINE__);                                                      \
-    } while (0)
 
 // cos/sin(2 pi e / n) with octant reduction (exact at multiples of n/8)
 static void twiddle_cs(long e, long n, double* c, double* s) {
@@ -329,6 +273,7 @@ static int launch_pass5(nsg_ctx* c, double2* uh, int b0, int nb, int stage, void
     a.k2 = (const char*)c->K[1] + b0 * c->sbox * e;
     a.k3 = (const char*)c->K[2] + b0 * c->sbox * e;
     a.kout = kout ? (char*)kout + b0 * c->sbox * e : nullptr;
+    a.extra = (MODE == EP_STAGE && c->closure && c->cnn) ? c->closure + b0 * c->sbox : nullptr;
     a.twN = twN_of<T>(c); a.sc = c->sc + b0; a.fslots = c->fslots + (long long)b0 * c->NK;
     a.kf2 = c->p.force_kf * c->p.force_kf; a.eps_in = c->p.force_eps; a.norm = norm;
     a.sbox = c->sbox; a.ibox = c->a4box;
@@ -416,6 +361,7 @@ static int check_boxes(nsg_ctx* c) {
 // ------------------------------------------------------------------------------------------
 static void free_all(nsg_ctx* c) {
     if (!c) return;
+    nsgi_closure_free(c);
     void* bufs[] = {c->uh, c->aux, c->K[0], c->K[1], c->K[2], c->B1, c->B2, c->twN64, c->twM64,
                     c->twN32, c->twM32, c->sc, c->fslots, c->part};
     for (void* b : bufs)
@@ -563,14 +509,33 @@ static int download_compact(nsg_ctx* c, const void* src, bool bulk, double* out)
     return NSG_OK;
 }
 
-// r2c of a real AoS field already in B2 into state box b (FP64, schedule 0).  A3/A4 of box b are
-// the scratch; the A4 region may overlap the AoS input, which pass 3 has consumed by then.
-template <int N> static int r2c_into_state(nsg_ctx* c, int b) {
+// r2c of a physical real AoS field [N^3][3] at src into the compact FP64 array dst of box b
+// (FP64, schedule 0).  A3/A4 of box b are the scratch: src must not lie in B1; it may be the
+// start of B2, which pass 3 has consumed before pass 4 writes A4 there.
+template <int N> static int r2c_to(nsg_ctx* c, const double* src, double2* dst, int b) {
     int rc;
-    if ((rc = launch_passz<double, N, 0, ZM_R2C>(c, c->B2, (double2*)c->B1 + b * c->a3box, 0, 0, 1, c->sc + b, 1.0, 0)))
+    if ((rc = launch_passz<double, N, 0, ZM_R2C>(c, src, (double2*)c->B1 + b * c->a3box, 0, 0, 1, c->sc + b, 1.0, 0)))
         return rc;
     if ((rc = launch_passy<double, N, 0, -1>(c, b, 1, 3))) return rc;
-    return launch_pass5<double, N, 0, EP_STORE>(c, c->uh, b, 1, 1, nullptr, 0, 1.0);
+    return launch_pass5<double, N, 0, EP_STORE>(c, dst, b, 1, 1, nullptr, 0, 1.0);
+}
+template <int N> static int r2c_into_state(nsg_ctx* c, int b) { return r2c_to<N>(c, (const double*)c->B2, c->uh, b); }
+
+// c2r of box b's compact FP64 3-vector (pointer to box 0 of its array) to a physical real AoS
+// field at dst (FP64, schedule 0); A1 (B1) and A2 (B2) of box b are the scratch
+template <int N> static int c2r_from(nsg_ctx* c, const double2* src0, int b, double* dst) {
+    int rc;
+    if ((rc = launch_pass1<double, N, 0>(c, src0, b, 1, nullptr, 1, 1, 0))) return rc;
+    if ((rc = launch_passy<double, N, 0, +1>(c, b, 1, 3))) return rc;
+    const double N3 = (double)N * N * N;
+    return launch_passz<double, N, 0, ZM_REAL>(c, (const double2*)c->B2 + b * c->a2box, dst, 0, 0, 1, c->sc + b, 1.0 / N3, 0);
+}
+
+int nsgi_c2r(nsg_ctx* c, const double2* src, double* dst) {
+    return dispatch_N(c->N, [&](auto n) { return c2r_from<decltype(n)::value>(c, src, 0, dst); });
+}
+int nsgi_r2c(nsg_ctx* c, const double* src, double2* dst) {
+    return dispatch_N(c->N, [&](auto n) { return r2c_to<decltype(n)::value>(c, src, dst, 0); });
 }
 
 static int init_state_box(nsg_ctx* c, int b, int kind, double k0, double energy, uint64_t seed) {
@@ -749,8 +714,17 @@ int nsg_rhs(nsg_ctx* c, const double* u_hat, double* out) {
 int nsg_step(nsg_ctx* c, long nsteps) {
     if (!c || nsteps < 0) return NSG_ERR_INVALID;
     HIPCHK(c, hipSetDevice(c->p.device));
-    const int rc = enqueue_steps(c, nsteps, 0, c->nbox);
-    if (rc) return rc;
+    int rc;
+    if (c->cnn) {
+        // learned closure: the model term is evaluated from the state at the start of each step
+        // and held fixed through its four stages
+        for (long i = 0; i < nsteps; ++i) {
+            if ((rc = nsgi_closure_eval(c))) return rc;
+            if ((rc = enqueue_steps(c, 1, 0, c->nbox))) return rc;
+        }
+    } else if ((rc = enqueue_steps(c, nsteps, 0, c->nbox))) {
+        return rc;
+    }
     if (nsteps == 0) {
         HIPCHK(c, hipStreamSynchronize(c->s));
         return NSG_OK;

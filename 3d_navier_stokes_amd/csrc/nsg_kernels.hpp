@@ -659,6 +659,7 @@ struct P5Args {
     const void* k2;
     const void* k3;
     void* kout;       // K_s output (stage < 4)
+    const double2* extra;  // optional term added to every stage's nonlinear term (state layout)
     const void* twN;
     const DevScalars* sc;
     const double* fslots;
@@ -683,6 +684,7 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
     const C* K2 = (const C*)a.k2 + box * a.sbox;
     const C* K3 = (const C*)a.k3 + box * a.sbox;
     C* ko = a.kout ? (C*)a.kout + box * a.sbox : nullptr;
+    const double2* extra = a.extra ? a.extra + box * a.sbox : nullptr;
     const double* fslots = a.fslots + box * a.NK;
     const int tid = threadIdx.x, w = tid % W, t = tid / W;
     const int kz = tile * W + w;
@@ -760,6 +762,14 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
                 }
                 nl[c].x += ffac * us.x;
                 nl[c].y += ffac * us.y;
+            }
+        }
+        if (extra && keep) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double2 e = extra[c * plane + idx];
+                nl[c].x += e.x;
+                nl[c].y += e.y;
             }
         }
         if (a.stage < 4) {

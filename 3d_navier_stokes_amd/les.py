@@ -1,0 +1,247 @@
+"""Learned-closure LES pipeline on one GPU, every step through libnsg.so.
+
+1. Forced DNS (constant-power forcing eps_in on 0 < |k| <= 2.5, CFL-adaptive RK4, FP32-bulk) from
+   a k^4 random-phase start.  The spin-up to the statistically steady state runs on a coarser
+   grid, is resampled onto the DNS grid and settles there; the run is then sampled every `every`
+   steps: each sample filters the DNS state onto the LES grid (Gaussian, cutoff k_c, plus the LES
+   grid's 2/3 cut) and returns the network input grad(ub) and the target, the deviatoric subgrid
+   stress; the filtered energy spectrum of every sample is kept as the reference.
+2. Training of the closure network on those samples (BF16 MFMA forward/backward, Adam), the last
+   samples held out.  The model's mean subgrid dissipation -<tau_ij S_ij> is compared with the
+   true one on the training samples; their ratio is the gain of a calibrated copy of the model.
+3. LES runs on the coarse grid from the last filtered DNS state, same forcing and viscosity: with
+   the trained closure, with its calibrated copy, without a model (the plain coarse DNS, the
+   fallback), and without a model at the constant eddy viscosity
+   nu_eff = eps_in (2 pi)^3 / (2 Omega_f) that balances the injection with the filtered field's
+   enstrophy Omega_f (a volume integral, like every measure here).  Each run's time-averaged E(k)
+   is compared with the filtered DNS E(k) on 1 <= k <= k_c.
+
+Usage (one GPU):  python 3d_navier_stokes_amd/les.py --dns 512 --les 64 --out report.json
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def load_nsg():
+    if "navier_stokes_amd" in sys.modules:
+        return sys.modules["navier_stokes_amd"]
+    spec = importlib.util.spec_from_file_location("navier_stokes_amd", os.path.join(HERE, "__init__.py"),
+                                                  submodule_search_locations=[HERE])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["navier_stokes_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def forced(nsg, N, nu, precision, eps=0.1, kf=2.5, cfl=0.5):
+    return nsg.Solver(N, nu=nu, dt=1e-3, precision=precision, cfl=cfl, force_eps=eps, force_kf=kf)
+
+
+def generate(nsg, Nd, Nl, kc, nu, spin_n, spin_time, settle, nsamples, every, seed=2, precision="fp32bulk"):
+    """forced DNS -> (samples [(x, tau)], mean filtered E(k), last filtered state, stats).  The
+    statistically steady state is reached cheaply: the run starts on a spin_n^3 grid until
+    t >= spin_time, is resampled (zero-padded) onto the Nd^3 grid and settles for `settle` steps
+    before sampling starts."""
+    t0 = time.perf_counter()
+    spin = forced(nsg, spin_n, nu, precision)
+    spin.init("k4", k0=2.0, energy=0.5, seed=seed)
+    nspin = 0
+    while spin.time()[0] < spin_time:
+        spin.step(100)
+        nspin += 100
+    dns = forced(nsg, Nd, nu, precision)
+    spin.resample_to(dns)
+    spin.close()
+    dns.step(settle)
+    t_spin = time.perf_counter() - t0
+    log(f"[les] spin-up {nspin} steps at {spin_n}^3 + {settle} at {Nd}^3: {t_spin:.1f} s, t = {dns.time()[0]:.3f}")
+    les = nsg.Solver(Nl, nu=nu)
+    samples, spectra, enst, energy = [], [], [], []
+    t_dns, n_dns, t_phys0 = 0.0, 0, dns.time()[0]
+    for s in range(nsamples):
+        t0 = time.perf_counter()
+        dns.step(every)
+        t_dns += time.perf_counter() - t0
+        n_dns += every
+        x, tau = dns.sgs_sample(les, kc)
+        tot, ek, _ = les.measures()
+        samples.append((x, tau))
+        spectra.append(ek)
+        enst.append(tot.enstrophy)
+        energy.append(dns.measures()[0].energy)
+    t_phys = dns.time()[0] - t_phys0
+    ub = les.get_state()
+    stats = {"spinup_steps_coarse": nspin, "spinup_grid": spin_n, "settle_steps": settle, "spinup_s": t_spin,
+             "dns_ms_per_step": 1e3 * t_dns / n_dns, "dns_mean_dt": t_phys / n_dns,
+             "dns_gpu_s_per_time_unit": t_dns / t_phys, "sampled_time_units": t_phys,
+             "filtered_enstrophy": float(np.mean(enst)), "dns_energy_first_last": [energy[0], energy[-1]]}
+    dns.close()
+    les.close()
+    return samples, np.mean(spectra, axis=0), ub, stats
+
+
+def sgs_dissipation(x, tau):
+    """Pi = -<tau_ij S_ij>, S = sym(grad ub); x [n, 9] (x[3 i + j] = d_j u_i), tau [n, 6]"""
+    g = x.astype(np.float64).reshape(-1, 3, 3)
+    S = 0.5 * (g + g.transpose(0, 2, 1))
+    t = tau.astype(np.float64)
+    return -float(np.mean(t[:, 0] * S[:, 0, 0] + t[:, 1] * S[:, 1, 1] + t[:, 2] * S[:, 2, 2]
+                          + 2 * (t[:, 3] * S[:, 0, 1] + t[:, 4] * S[:, 0, 2] + t[:, 5] * S[:, 1, 2])))
+
+
+def train(nsg, samples, Nl, steps, batch, lr, holdout, seed=1):
+    xs = np.stack([s[0] for s in samples])
+    ts = np.stack([s[1] for s in samples])
+    sig_in = float(np.sqrt(np.mean(xs.astype(np.float64) ** 2)))
+    sig_out = [float(v) for v in np.sqrt(np.mean(ts.astype(np.float64) ** 2, axis=(0, 1)))]
+    net = nsg.Network(Nl, max_batch=batch, seed=seed)
+    net.set_norm(sig_in, sig_out)
+    ntrain = len(samples) - holdout
+    for x, t in samples:
+        net.add_sample(x, t)
+    l_train0 = net.eval(0, ntrain)
+    l_hold0 = net.eval(ntrain, holdout) if holdout else None
+    t0 = time.perf_counter()
+    hist = []
+    chunk = 50
+    done = 0
+    while done < steps:
+        k = min(chunk, steps - done)
+        rate = lr * (0.1 if done >= 0.8 * steps else 1.0)
+        hist.extend(net.train(k, batch=batch, lr=rate, seed=seed + done, first=0, count=ntrain).tolist())
+        done += k
+    t_train = time.perf_counter() - t0
+    res = {"sigma_in": sig_in, "sigma_out": sig_out, "train_samples": ntrain, "holdout_samples": holdout,
+           "loss_train_initial": l_train0, "loss_train_final": net.eval(0, ntrain),
+           "loss_holdout_initial": l_hold0, "loss_holdout_final": net.eval(ntrain, holdout) if holdout else None,
+           "steps": steps, "batch": batch, "lr": lr, "train_s": t_train,
+           "loss_history_every_50": hist[::50]}
+    # SGS dissipation of the model against the truth on the training samples; the gain that
+    # matches them calibrates the output (a-priori, training data only)
+    pi_true, pi_model = 0.0, 0.0
+    for x, t in samples[:ntrain]:
+        y = net.forward((x / sig_in)[None])[0] * np.array(sig_out)
+        pi_true += sgs_dissipation(x, t)
+        pi_model += sgs_dissipation(x, y)
+    res.update({"sgs_dissipation_true": pi_true / ntrain, "sgs_dissipation_model": pi_model / ntrain,
+                "dissipation_gain": pi_true / pi_model if pi_model > 0 else 1.0})
+    return net, res
+
+
+def scaled_network(nsg, net, gain, Nl, batch):
+    """a copy of `net` whose output is multiplied by `gain` (last layer's weights and bias)"""
+    p = net.get_params().astype(np.float64)
+    n_last = 6 * 32 * 27 + 6
+    p[-n_last:] *= gain
+    out = nsg.Network(Nl, max_batch=batch)
+    si, so = net.norm
+    out.set_norm(si, so)
+    out.set_params(p.astype(np.float32))
+    return out
+
+
+def run_les(nsg, Nl, ub, nu, steps, net=None, avg_from=0.2, eps=0.1):
+    s = forced(nsg, Nl, nu, "fp64")
+    s.set_state(ub)
+    if net is not None:
+        s.attach_closure(net)
+    spectra, t0w = [], time.perf_counter()
+    t_start = s.time()[0]
+    blow = None
+    nchunk = 50
+    for i in range(0, steps, nchunk):
+        try:
+            s.step(nchunk)
+        except nsg.NSGError as e:
+            blow = f"step {i}: {e}"
+            break
+        if i >= avg_from * steps:
+            spectra.append(s.measures()[1])
+    wall = time.perf_counter() - t0w
+    t_phys = s.time()[0] - t_start
+    tot = s.measures()[0] if blow is None else None
+    s.close()
+    ek = np.mean(spectra, axis=0) if spectra else None
+    return {"E_k": ek, "blowup": blow, "gpu_s": wall, "time_units": t_phys,
+            "ms_per_step": 1e3 * wall / steps, "final_energy": tot.energy if tot else None}
+
+
+def spectral_error(ek, ref, kc):
+    k = np.arange(1, int(kc) + 1)
+    if ek is None:
+        return None, None
+    r = np.abs(ek[k] - ref[k]) / ref[k]
+    return float(np.max(r)), float(np.mean(r))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--dns", type=int, default=512)
+    ap.add_argument("--les", type=int, default=64)
+    ap.add_argument("--kc", type=float, default=None, help="filter cutoff (default les // 3)")
+    ap.add_argument("--nu", type=float, default=1e-3)
+    ap.add_argument("--spin-n", type=int, default=256, help="grid of the spin-up run")
+    ap.add_argument("--spin-time", type=float, default=20.0, help="spin-up length in time units")
+    ap.add_argument("--settle", type=int, default=1500, help="steps on the DNS grid before sampling")
+    ap.add_argument("--samples", type=int, default=64)
+    ap.add_argument("--every", type=int, default=60)
+    ap.add_argument("--holdout", type=int, default=8)
+    ap.add_argument("--train-steps", type=int, default=1500)
+    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--les-steps", type=int, default=5000)
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args()
+    kc = a.kc if a.kc else a.les // 3
+    nsg = load_nsg()
+    samples, ek_ref, ub, dstats = generate(nsg, a.dns, a.les, kc, a.nu, a.spin_n, a.spin_time, a.settle, a.samples,
+                                           a.every)
+    log(f"[les] {len(samples)} samples; DNS {dstats['dns_ms_per_step']:.2f} ms/step")
+    net, tstats = train(nsg, samples, a.les, a.train_steps, a.batch, a.lr, a.holdout)
+    log(f"[les] training: loss {tstats['loss_train_initial']:.4f} -> {tstats['loss_train_final']:.4f}, holdout "
+        f"{tstats['loss_holdout_initial']} -> {tstats['loss_holdout_final']}, {tstats['train_s']:.1f} s")
+    # measures are volume integrals over (2 pi)^3 while eps_in is a volume average
+    nu_eff = 0.1 * (2.0 * np.pi) ** 3 / (2.0 * dstats["filtered_enstrophy"])
+    cal = scaled_network(nsg, net, tstats["dissipation_gain"], a.les, a.batch)
+    runs = {}
+    for name, net_, nu_ in [("cnn_closure", net, a.nu), ("cnn_closure_dissipation_calibrated", cal, a.nu),
+                            ("no_model", None, a.nu), ("eddy_viscosity", None, nu_eff)]:
+        r = run_les(nsg, a.les, ub, nu_, a.les_steps, net_)
+        emax, emean = spectral_error(r["E_k"], ek_ref, kc)
+        r.update({"nu": nu_, "E_k_max_rel_err_k_le_kc": emax, "E_k_mean_rel_err_k_le_kc": emean})
+        log(f"[les] {name}: max rel E(k) err {emax}, mean {emean}, {r['ms_per_step']:.3f} ms/step, blowup {r['blowup']}")
+        runs[name] = r
+    les_s_per_tu = runs["cnn_closure"]["gpu_s"] / max(runs["cnn_closure"]["time_units"], 1e-30)
+    report = {
+        "config": {"dns": a.dns, "les": a.les, "kc": kc, "nu": a.nu, "forcing": "eps_in=0.1 on |k|<=2.5",
+                   "spin_n": a.spin_n, "spin_time": a.spin_time, "settle": a.settle, "samples": a.samples,
+                   "sample_every": a.every,
+                   "les_steps": a.les_steps, "dns_precision": "fp32bulk", "les_precision": "fp64"},
+        "dns": dstats, "training": tstats,
+        "filtered_dns_E_k": ek_ref[: int(kc) + 2].tolist(),
+        "les": {k: {kk: (vv.tolist() if isinstance(vv, np.ndarray) else vv) for kk, vv in v.items()} for k, v in runs.items()},
+        "gpu_time_ratio_dns_over_les_per_time_unit": dstats["dns_gpu_s_per_time_unit"] / les_s_per_tu,
+    }
+    line = json.dumps(report)
+    if a.out:
+        with open(a.out, "w") as f:
+            f.write(line + "\n")
+    print(line)
+
+
+if __name__ == "__main__":
+    main()

@@ -143,6 +143,58 @@ int nsg_ensemble_autotune(nsg_ensemble* e, int iters, int use_cache, int* chosen
 int nsg_ensemble_set_schedule(nsg_ensemble* e, int group, int sched);
 double nsg_ensemble_step_bytes(const nsg_ensemble* e);
 
+/* Learned subgrid-stress closure network (no reference counterpart: the reference has no LES).
+ * Four periodic 3x3x3 convolutions 9 -> 32 -> 32 -> 32 -> 6 channels with exact GELU between
+ * them, on an N^3 grid (N a multiple of 16); BF16 weights and activations, FP32 accumulation on
+ * the matrix cores.  Input: the 9 velocity-gradient components g[3 i + j] = d_j u_i of the
+ * filtered field; output: the deviatoric subgrid stress (xx, yy, zz, xy, xz, yz).  Host arrays
+ * are channels-last float [batch][N^3][C], point index (x N + y) N + z.  Parameters are
+ * exchanged as floats in torch conv3d order, layer by layer: W[co][ci][3][3][3] then b[co].
+ * nsg_cnn_forward / nsg_cnn_loss_grad take normalised inputs and targets; nsg_cnn_add_sample
+ * takes physical ones and divides by the normalisation set with nsg_cnn_set_norm (inputs by
+ * sigma_in, stress component c by sigma_out[c]).  Loss: mean squared error over points and the
+ * 6 outputs; training: Adam (beta 0.9, 0.999, eps 1e-8) on FP32 master weights. */
+typedef struct nsg_cnn nsg_cnn;
+long long nsg_cnn_param_count(void);
+int nsg_cnn_create(int N, int max_batch, uint64_t seed, int device, nsg_cnn** out);
+void nsg_cnn_destroy(nsg_cnn* c);
+const char* nsg_cnn_last_error(const nsg_cnn* c);
+int nsg_cnn_set_params(nsg_cnn* c, const float* params);
+int nsg_cnn_get_params(nsg_cnn* c, float* params);
+int nsg_cnn_set_norm(nsg_cnn* c, double sigma_in, const double* sigma_out);
+int nsg_cnn_get_norm(const nsg_cnn* c, double* sigma_in, double* sigma_out);
+int nsg_cnn_forward(nsg_cnn* c, const float* x, int batch, float* y);
+/* loss and its gradient with respect to every parameter (grads may be NULL: loss only) */
+int nsg_cnn_loss_grad(nsg_cnn* c, const float* x, const float* target, int batch, double* loss, float* grads);
+/* training set resident on the device: one sample = one N^3 field pair */
+int nsg_cnn_add_sample(nsg_cnn* c, const float* x, const float* tau);
+int nsg_cnn_sample_count(const nsg_cnn* c);
+/* `steps` Adam steps on random batches drawn from samples [first, first + count) (indices from
+ * `seed`); loss_per_step may be NULL */
+int nsg_cnn_train(nsg_cnn* c, int first, int count, int steps, int batch, double lr, uint64_t seed, double* loss_per_step);
+/* mean loss over samples [first, first + count) */
+int nsg_cnn_eval(nsg_cnn* c, int first, int count, double* loss);
+/* mean time (ms, HIP events) of one inference pass over `batch` fields */
+int nsg_cnn_time_forward(nsg_cnn* c, int batch, int iters, double* ms);
+
+/* LES with the learned closure (no reference counterpart).  Filter: Gaussian
+ * G(k) = exp(-k^2 Delta^2 / 24), Delta = pi / kc, times the LES grid's 2/3 cut |k| <= N_les / 3.
+ * nsg_sgs_sample: from a DNS context's state, writes the filtered state into the (coarser) LES
+ * context and, when the pointers are not NULL, the model input x = grad(u_filtered)
+ * [N_les^3][9] (x[3 i + j] = d_j u_i) and the target: the deviatoric part of
+ * tau_ij = F(u_i u_j) - T(ub_i ub_j) [N_les^3][6] (F: the filter, T: the sharp cut).
+ * nsg_attach_closure: every later nsg_step of the (single-box) context adds
+ * -P div(tau_model(grad u)) to the nonlinear term of each RK stage, evaluated from the state at
+ * the start of the step; NULL detaches.  The network must live as long as it is attached.
+ * nsg_closure_term: that term for the current state, reference AoS half-spectrum layout. */
+int nsg_sgs_sample(nsg_ctx* dns, nsg_ctx* les, double kc, float* x_out, float* tau_out);
+/* spectral resampling between grids (truncation to, or zero-padding from, the coarser grid's
+ * 2/3 sphere), FFTW normalisation rescaled by (N_dst / N_src)^3; time copied, dt scaled by
+ * N_src / N_dst (the CFL ratio) */
+int nsg_resample(nsg_ctx* src, nsg_ctx* dst);
+int nsg_attach_closure(nsg_ctx* ctx, nsg_cnn* net);
+int nsg_closure_term(nsg_ctx* ctx, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,112 @@
+"""CPU checks of the closure oracle (oracle/closure_oracle.py) before it is trusted as the
+checker of the GPU closure path.
+
+Pins:
+  * network forward, MSE loss and every parameter gradient against torch (CPU, float64,
+    autograd) golden vectors (tests/golden/cnn.npz, tests/golden/make_cnn_golden.py);
+  * the hand-written backward against central finite differences of the oracle's own loss;
+  * bfloat16 rounding against exact binary values (ties to even, NaN kept);
+  * the subgrid stress against the closed form for u = (0, 0, A sin x):
+    tau_zz = A^2/2 (1 - G1^2) - A^2/2 (G2 - G1^2) cos 2x, G_k = exp(-k^2 Delta^2 / 24);
+  * -P div(u u) against the C oracle's nonlinear term P(u x w) for a band-limited field
+    (the two differ by a gradient, which P removes).
+"""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT, rel
+
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import closure_oracle as C  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def cnn_golden():
+    return np.load(os.path.join(GOLDEN, "cnn.npz"))
+
+
+def test_forward_loss_grads_vs_torch(cnn_golden):
+    g = cnn_golden
+    params = C.unflatten(g["params"])
+    y, _ = C.forward(params, g["x"])
+    assert rel(y, g["y"]) < 1e-12
+    loss, grads = C.loss_and_grads(params, g["x"], g["t"])
+    assert abs(loss - float(g["loss"])) < 1e-12 * float(g["loss"])
+    flat = np.concatenate([np.concatenate([dW.ravel(), db.ravel()]) for dW, db in grads])
+    assert rel(flat, g["grads"]) < 1e-10
+
+
+def test_backward_finite_differences():
+    rng = np.random.default_rng(3)
+    ch = (2, 3, 3, 2)
+    params = C.init_params(5, ch)
+    x = rng.standard_normal((1, 4, 4, 4, ch[0]))
+    t = rng.standard_normal((1, 4, 4, 4, ch[-1]))
+    loss, grads = C.loss_and_grads(params, x, t)
+    flat = C.flatten(params)
+    gflat = np.concatenate([np.concatenate([dW.ravel(), db.ravel()]) for dW, db in grads])
+    h = 1e-6
+    for i in rng.choice(flat.size, 40, replace=False):
+        p = flat.copy()
+        p[i] += h
+        lp, _ = C.loss_and_grads(C.unflatten(p, ch), x, t)
+        p[i] -= 2 * h
+        lm, _ = C.loss_and_grads(C.unflatten(p, ch), x, t)
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - gflat[i]) < 1e-6 * max(1.0, abs(fd)), (i, fd, gflat[i])
+
+
+def test_bf16_rounding():
+    one = 1.0
+    assert C.bf16(np.array([one]))[0] == 1.0
+    assert C.bf16(np.array([1 + 2.0 ** -8]))[0] == 1.0  # tie -> even
+    assert C.bf16(np.array([1 + 3 * 2.0 ** -8]))[0] == 1 + 2.0 ** -6  # tie -> even (up)
+    assert C.bf16(np.array([1 + 2.0 ** -8 + 2.0 ** -12]))[0] == 1 + 2.0 ** -7  # above the tie
+    assert C.bf16(np.array([-3.0]))[0] == -3.0
+    assert np.isnan(C.bf16(np.array([np.nan]))[0])
+
+
+def test_sgs_single_mode_closed_form():
+    N, Nl, kc, A = 24, 12, 4.0, 0.7
+    x = 2 * np.pi * np.arange(N) / N
+    X = np.meshgrid(x, x, x, indexing="ij")[0]
+    u = np.zeros((N, N, N, 3))
+    u[..., 2] = A * np.sin(X)
+    uh = C.r2c(u)
+    ub, g, tau = C.sgs_sample(uh, N, Nl, kc)
+    d2 = (math.pi / kc) ** 2
+    G1, G2 = math.exp(-d2 / 24), math.exp(-4 * d2 / 24)
+    xl = 2 * np.pi * np.arange(Nl) / Nl
+    Xl = np.meshgrid(xl, xl, xl, indexing="ij")[0].reshape(-1)
+    tzz = A * A / 2 * (1 - G1 ** 2) - A * A / 2 * (G2 - G1 ** 2) * np.cos(2 * Xl)
+    want = np.zeros((Nl ** 3, 6))
+    want[:, 0] = want[:, 1] = -tzz / 3
+    want[:, 2] = 2 * tzz / 3
+    assert np.max(np.abs(tau - want)) < 1e-13
+    gw = np.zeros((Nl ** 3, 9))
+    gw[:, 6] = G1 * A * np.cos(Xl)  # d_x ub_z
+    assert np.max(np.abs(g - gw)) < 1e-13
+    ubr = C.c2r(ub, Nl).reshape(-1, 3)
+    assert np.max(np.abs(ubr[:, 2] - G1 * A * np.sin(Xl))) < 1e-13
+
+
+def test_closure_term_equals_nonlinear_term(oracle):
+    """-P div(u u) == P(u x w) when u is band-limited to |k| <= N/6 (no aliasing)"""
+    N = 24
+    rng = np.random.default_rng(11)
+    uh = oracle.ic_k4(N, 2.0, 0.5, 7)
+    kx, ky, kz = C.wavenumbers(N)
+    uh = uh * ((kx * kx + ky * ky + kz * kz) <= 9)[..., None]
+    u = C.c2r(uh, N).reshape(-1, 3)
+    tau = np.stack([u[:, i] * u[:, j] for i, j in C.TAU_ORDER], axis=-1)
+    M = C.closure_term(tau, N)
+    want = oracle.rhs(uh)
+    assert rel(M, want) < 1e-12
+    # an isotropic stress is a pure gradient after divergence: no closure term
+    iso = np.zeros_like(tau)
+    iso[:, 0] = iso[:, 1] = iso[:, 2] = rng.standard_normal(N ** 3)
+    assert np.max(np.abs(C.closure_term(iso, N))) < 1e-10

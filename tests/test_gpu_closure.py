@@ -1,0 +1,154 @@
+"""Learned closure on the GPU (BF16 MFMA network kernels, filter / subgrid-stress sampling, LES
+closure term) through the C ABI, against the CPU closure oracle (oracle/closure_oracle.py).
+
+Bars:
+  * network forward vs the oracle with the same bfloat16 roundings: rel L2 <= 5e-3 (FP32
+    accumulation order and float GELU can move a bfloat16 rounding by one unit, 2^-8);
+  * loss rel <= 1e-3, every layer's weight / bias gradient rel L2 <= 2e-2 vs the emulating oracle;
+  * filtered state, gradients and subgrid stress vs the FP64 oracle: rel L2 <= 1e-6 (outputs are
+    float32);
+  * closure term vs the oracle chain (gradients -> network -> -P div tau): rel L2 <= 1e-2.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, rel
+
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import closure_oracle as C  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+NG = 16
+
+
+def rand_inputs(seed, B, N=NG):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((B, N, N, N, 9)).astype(np.float32)
+    t = (0.3 * rng.standard_normal((B, N, N, N, 6))).astype(np.float32)
+    return x, t
+
+
+def oracle_params(net):
+    return C.unflatten(net.get_params().astype(np.float64))
+
+
+def test_param_count(nsg):
+    with nsg.Network(NG) as net:
+        assert net.param_count == C.flatten(C.init_params(0)).size == 68358
+        p = net.get_params()
+        bound = 1 / np.sqrt(27 * 9)
+        assert np.all(np.abs(p[:32 * 9 * 27]) <= bound)
+        q = np.random.default_rng(1).standard_normal(p.size).astype(np.float32) * 0.05
+        net.set_params(q)
+        assert np.array_equal(net.get_params(), q)
+
+
+@pytest.mark.parametrize("N,B", [(16, 1), (16, 2), (32, 1)])
+def test_forward_vs_oracle(nsg, N, B):
+    x, _ = rand_inputs(10 + N + B, B, N)
+    with nsg.Network(N, max_batch=B, seed=5) as net:
+        got = net.forward(x).reshape(B, N, N, N, 6)
+        params = oracle_params(net)
+    want, _ = C.forward(params, x.astype(np.float64), emulate_bf16=True)
+    assert rel(got, want) < 5e-3
+    exact, _ = C.forward(params, x.astype(np.float64))
+    assert rel(got, exact) < 3e-2
+
+
+def test_loss_and_gradients_vs_oracle(nsg):
+    B = 2
+    x, t = rand_inputs(21, B)
+    with nsg.Network(NG, max_batch=B, seed=9) as net:
+        loss, g = net.loss_grad(x, t)
+        params = oracle_params(net)
+    lo, go = C.loss_and_grads(params, x.astype(np.float64), t.astype(np.float64), emulate_bf16=True)
+    assert abs(loss - lo) < 1e-3 * lo
+    off = 0
+    for l, (dW, db) in enumerate(go):
+        gw = g[off:off + dW.size].reshape(dW.shape)
+        off += dW.size
+        gb = g[off:off + db.size]
+        off += db.size
+        assert rel(gw, dW) < 2e-2, f"layer {l} weights"
+        assert rel(gb, db) < 2e-2, f"layer {l} bias"
+
+
+def test_training_reduces_loss(nsg):
+    """fit a teacher network's output: the loss falls by more than 2x in 150 Adam steps"""
+    B = 2
+    x, _ = rand_inputs(31, B)
+    teacher = C.init_params(77)
+    t, _ = C.forward(teacher, x.astype(np.float64))
+    t = (t / t.std()).astype(np.float32)
+    with nsg.Network(NG, max_batch=B, seed=2) as net:
+        net.set_norm(1.0, [1.0] * 6)
+        for b in range(B):
+            net.add_sample(x[b].reshape(-1, 9), t[b].reshape(-1, 6))
+        l0 = net.eval()
+        hist = net.train(150, batch=2, lr=2e-3, seed=4)
+        l1 = net.eval()
+    assert np.all(np.isfinite(hist))
+    assert l1 < l0 / 2, (l0, l1)
+
+
+@pytest.mark.parametrize("Nd,Nl,kc", [(48, 24, 8.0), (64, 32, 10.0)])
+def test_sgs_sample_vs_oracle(nsg, oracle, Nd, Nl, kc):
+    uh = oracle.ic_k4(Nd, 3.0, 0.5, 17)
+    with nsg.Solver(Nd) as dns, nsg.Solver(Nl) as les:
+        dns.set_state(uh)
+        x, tau = dns.sgs_sample(les, kc)
+        ub = les.get_state()
+    ubo, xo, tauo = C.sgs_sample(uh, Nd, Nl, kc)
+    assert rel(ub, ubo) < 1e-12
+    assert rel(x, xo) < 1e-6
+    assert rel(tau, tauo) < 1e-6
+
+
+def test_closure_term_vs_oracle(nsg, oracle):
+    N = 32
+    uh = oracle.ic_k4(N, 3.0, 0.5, 23)
+    sig_in, sig_out = 2.5, [0.05, 0.05, 0.05, 0.03, 0.03, 0.03]
+    with nsg.Network(N, seed=8) as net, nsg.Solver(N, nu=1e-3, dt=1e-3) as les:
+        net.set_norm(sig_in, sig_out)
+        les.set_state(uh)
+        les.attach_closure(net)
+        M = les.closure_term()
+        params = oracle_params(net)
+    g = C.curl_free_grad(uh, N).reshape(1, N, N, N, 9) / sig_in
+    y, _ = C.forward(params, g, emulate_bf16=True)
+    tau = y.reshape(-1, 6) * np.array(sig_out)
+    want = C.closure_term(tau, N)
+    assert rel(M, want) < 1e-2
+
+
+def test_les_step_with_closure(nsg, oracle):
+    """zero network: the step is bitwise the plain step; trained-free random network: the state
+    change differs from the plain step's by dt M to first order"""
+    N, dt = 32, 1e-3
+    uh = oracle.ic_k4(N, 3.0, 0.5, 29)
+    with nsg.Solver(N, nu=2e-3, dt=dt) as plain:
+        plain.set_state(uh)
+        plain.step(1)
+        u_plain = plain.get_state()
+    with nsg.Network(N, seed=3) as net, nsg.Solver(N, nu=2e-3, dt=dt) as les:
+        net.set_norm(2.5, [0.05] * 6)
+        zero = np.zeros(net.param_count, dtype=np.float32)
+        net.set_params(zero)
+        les.attach_closure(net)
+        les.set_state(uh)
+        les.step(1)
+        assert np.array_equal(les.get_state(), u_plain)
+        net.set_params(np.random.default_rng(4).standard_normal(net.param_count).astype(np.float32) * 0.02)
+        les.set_state(uh)
+        M = les.closure_term()
+        les.step(1)
+        d = les.get_state() - u_plain
+        assert rel(d, dt * M) < 0.05
+        les.attach_closure(None)
+        les.set_state(uh)
+        les.step(1)
+        assert np.array_equal(les.get_state(), u_plain)
