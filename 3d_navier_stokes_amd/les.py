@@ -85,7 +85,11 @@ def generate(nsg, Nd, Nl, kc, nu, spin_n, spin_time, settle, nsamples, every, se
         energy.append(dns.measures()[0].energy)
     t_phys = dns.time()[0] - t_phys0
     ub = les.get_state()
-    stats = {"spinup_steps_coarse": nspin, "spinup_grid": spin_n, "settle_steps": settle, "spinup_s": t_spin,
+    h = len(spectra) // 2
+    e1, e2 = np.mean(spectra[:h], axis=0), np.mean(spectra[h:], axis=0)
+    kk = np.arange(1, int(kc) + 1)
+    stats = {"filtered_E_k_half_window_rel_diff": (np.abs(e1[kk] - e2[kk]) / (0.5 * (e1[kk] + e2[kk]))).tolist(),
+             "spinup_steps_coarse": nspin, "spinup_grid": spin_n, "settle_steps": settle, "spinup_s": t_spin,
              "dns_ms_per_step": 1e3 * t_dns / n_dns, "dns_mean_dt": t_phys / n_dns,
              "dns_gpu_s_per_time_unit": t_dns / t_phys, "sampled_time_units": t_phys,
              "filtered_enstrophy": float(np.mean(enst)), "dns_energy_first_last": [energy[0], energy[-1]]}

@@ -15,6 +15,11 @@ evaluation at 512^3, scaled to steps/s as 1/(4 x that time).
 "ensemble_96_boxes" reports the batched-ensemble workload: 96 independent FP32-bulk boxes (32 at
 32^3, 32 at 48^3, 24 at 64^3, 8 at 96^3), 100 RK4 steps, autotuned FFT schedules, one HIP graph
 replay per ensemble step, against the same steps launched box by box.
+
+"closure_les" times the learned-closure LES on the 64^3 grid (the forced DNS state filtered with
+cutoff 21, the BF16 MFMA network evaluated every step) and reports the ratio of GPU seconds per
+unit of simulated time, DNS over LES, plus the network's forward throughput against the MI355X
+dense BF16 peak.
 """
 from __future__ import annotations
 
@@ -72,9 +77,11 @@ def run_mode(nsg, N, precision, steps, warmup, prof_iters):
     s.init("k4", k0=2.0, energy=0.5, seed=2)
     log(f"[bench] {precision}: warmup {warmup} steps")
     s.step(warmup)
+    tb = s.time()[0]
     t0 = time.perf_counter()
     s.step(steps)  # returns after the stream has drained
     el = time.perf_counter() - t0
+    mean_dt = (s.time()[0] - tb) / steps
     ms, by = s.profile(prof_iters)
     log(f"[bench] {precision}: {steps / el:.2f} steps/s; pass ms {['%.3f' % m for m in ms[:5]]}")
     passes = {}
@@ -88,7 +95,7 @@ def run_mode(nsg, N, precision, steps, warmup, prof_iters):
             "traffic": measured_traffic(N, precision, dom),
             "bytes_per_launch": by[dom], "ms_per_launch": ms[dom]}
     step_gb = s.step_bytes() / 1e9
-    out = {"steps_per_s": steps / el, "ms_per_step": 1e3 * el / steps, "roofline": roof, "passes": passes,
+    out = {"steps_per_s": steps / el, "ms_per_step": 1e3 * el / steps, "mean_dt": mean_dt, "roofline": roof, "passes": passes,
            "step_bytes_GB": step_gb, "step_GBps": step_gb / (el / steps)}
     return s, out
 
@@ -141,6 +148,47 @@ def run_ensemble(nsg, steps, warmup):
     return out
 
 
+CNN_CH = (9, 32, 32, 32, 6)
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def run_closure(nsg, dns, r_dns, Nl=64, steps=200, warmup=20):
+    """closure-LES cost against the DNS: the DNS state filtered onto the Nl^3 grid, the network
+    attached (its cost does not depend on the weights: small random ones keep the run tame), and
+    GPU seconds per unit of simulated time of both runs; plus the network's MFMA throughput"""
+    les = nsg.Solver(Nl, nu=1e-3, dt=1e-3, precision="fp64", cfl=0.5, force_eps=0.1, force_kf=2.5)
+    dns.sgs_sample(les, Nl // 3, want_x=False, want_tau=False)
+    t_dns, dt_dns, _ = dns.time()
+    les.set_time(t_dns, dt_dns * dns.N / Nl)
+    net = nsg.Network(Nl, seed=1)
+    net.set_norm(1.8, [0.035] * 6)
+    net.set_params(net.get_params() * 0.1)
+    les.attach_closure(net)
+    les.step(warmup)
+    tb = les.time()[0]
+    t0 = time.perf_counter()
+    les.step(steps)
+    el = time.perf_counter() - t0
+    dt_les = (les.time()[0] - tb) / steps
+    ms_fwd = net.time_forward(1, 20)
+    les.close()
+    flop = 2.0 * Nl ** 3 * 27 * sum(a * b for a, b in zip(CNN_CH[:-1], CNN_CH[1:]))
+    tflops = flop / (ms_fwd * 1e-3) / 1e12
+    dns_s_per_tu = r_dns["ms_per_step"] * 1e-3 / r_dns["mean_dt"]
+    les_s_per_tu = el / steps / dt_les
+    out = {"les_grid": Nl, "les_ms_per_step": 1e3 * el / steps, "les_mean_dt": dt_les,
+           "dns_ms_per_step": r_dns["ms_per_step"], "dns_mean_dt": r_dns["mean_dt"],
+           "gpu_time_ratio_dns_over_les": dns_s_per_tu / les_s_per_tu,
+           "cnn_forward_ms": ms_fwd, "cnn_forward_GFLOP": flop / 1e9,
+           "roofline": {"bound": "mfma", "kernel": "conv3_kernel (4 layers, one forward)", "achieved": round(tflops, 2),
+                        "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(tflops / BF16_PEAK_TFLOPS, 5),
+                        "traffic": None}}
+    log(f"[bench] closure LES {Nl}^3: {out['les_ms_per_step']:.3f} ms/step, GPU-time ratio DNS/LES "
+        f"{out['gpu_time_ratio_dns_over_les']:.0f}, CNN forward {ms_fwd:.3f} ms ({tflops:.1f} TFLOP/s)")
+    net.close()
+    return out
+
+
 def cpu_baseline(N):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O  # the checker; timed here only as the reported CPU baseline
@@ -166,6 +214,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=512)
     ap.add_argument("--ensemble-steps", type=int, default=100, help="0 skips the 96-box ensemble")
+    ap.add_argument("--no-closure", dest="closure", action="store_false", help="skip the closure-LES timing")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -205,6 +254,7 @@ def main():
     u32 = s32.get_state()
     state_err = float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64))
     s64.close()
+    clos = run_closure(nsg, s32, r32) if args.closure else None
     s32.close()
 
     ens = run_ensemble(nsg, args.ensemble_steps, 3) if args.ensemble_steps > 0 else None
@@ -244,6 +294,7 @@ def main():
         "step_bytes_GB": r32["step_bytes_GB"],
         "step_GBps": r32["step_GBps"],
         "ensemble_96_boxes": ens,
+        "closure_les": clos,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
