@@ -70,6 +70,10 @@ template <typename T, int N, int S> static constexpr int EYof() { return is64<T>
 template <typename T, int N, int S> static constexpr int EZof() { return is64<T>() ? sched_of(N, S).ez64 : sched_of(N, S).ez32; }
 template <typename T, int N, int S> static constexpr int WXof() { return is64<T>() ? sched_of(N, S).wx64 : sched_of(N, S).wx32; }
 template <typename T, int N, int S> static constexpr int WYof() { return is64<T>() ? sched_of(N, S).wy64 : sched_of(N, S).wy32; }
+// pass 5 holds three fields plus the epilogue operands: at 512^3 FP32 a 1024-thread block of 8
+// elements per thread spills under its 128-VGPR cap; 512 threads of 16 elements do not (measured
+// 0.51 -> 0.48 ms per launch)
+template <typename T, int N, int S> static constexpr int E5of() { return (!is64<T>() && N == 512) ? 16 : EXof<T, N, S>(); }
 
 template <typename F> static int dispatch_N(int N, F&& f) {
     switch (N) {
@@ -263,7 +267,7 @@ static int launch_passz(nsg_ctx* c, const void* in, void* out, long long ibox, l
 
 template <typename T, int N, int S, int MODE>
 static int launch_pass5(nsg_ctx* c, double2* uh, int b0, int nb, int stage, void* kout, int force, double norm) {
-    constexpr int EPT = EXof<T, N, S>(), W = WXof<T, N, S>(), TPF = N / EPT;
+    constexpr int EPT = E5of<T, N, S>(), W = WXof<T, N, S>(), TPF = N / EPT;
     static_assert(TPF * W <= 1024, "block too large");
     constexpr size_t e = esz<T>();
     P5Args a;

@@ -107,7 +107,43 @@ def sgs_dissipation(x, tau):
                           + 2 * (t[:, 3] * S[:, 0, 1] + t[:, 4] * S[:, 0, 2] + t[:, 5] * S[:, 1, 2])))
 
 
-def train(nsg, samples, Nl, steps, batch, lr, holdout, seed=1):
+def rotations():
+    """the 24 proper rotations of the cube (signed permutation matrices with det +1)"""
+    import itertools
+    out = []
+    for perm in itertools.permutations(range(3)):
+        for signs in itertools.product((1, -1), repeat=3):
+            R = np.zeros((3, 3), dtype=np.int64)
+            for a in range(3):
+                R[a, perm[a]] = signs[a]
+            if round(np.linalg.det(R)) == 1:
+                out.append(R)
+    return out
+
+
+TAU_ORDER = ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))
+
+
+def rotate_sample(x, tau, N, R):
+    """the sample seen in a frame rotated by R: fields move with the grid (x' = R x, periodic) and
+    transform as tensors (g' = R g R^T, tau' = R tau R^T)"""
+    q = np.indices((N, N, N)).reshape(3, -1)
+    p = (R.T @ q) % N  # source point of each destination point
+    src = (p[0] * N + p[1]) * N + p[2]
+    g = x.astype(np.float64)[src].reshape(-1, 3, 3)
+    g = np.einsum("ia,nab,jb->nij", R, g, R)
+    t6 = tau.astype(np.float64)[src]
+    T = np.empty((t6.shape[0], 3, 3))
+    for c, (i, j) in enumerate(TAU_ORDER):
+        T[:, i, j] = T[:, j, i] = t6[:, c]
+    T = np.einsum("ia,nab,jb->nij", R, T, R)
+    t_out = np.stack([T[:, i, j] for i, j in TAU_ORDER], axis=-1)
+    return g.reshape(-1, 9).astype(np.float32), t_out.astype(np.float32)
+
+
+def train(nsg, samples, Nl, steps, batch, lr, holdout, seed=1, augment=0):
+    """train on samples[:-holdout]; `augment` extra copies of each training sample in random cube
+    rotations (the closure has no preferred axis)"""
     xs = np.stack([s[0] for s in samples])
     ts = np.stack([s[1] for s in samples])
     sig_in = float(np.sqrt(np.mean(xs.astype(np.float64) ** 2)))
@@ -119,6 +155,13 @@ def train(nsg, samples, Nl, steps, batch, lr, holdout, seed=1):
         net.add_sample(x, t)
     l_train0 = net.eval(0, ntrain)
     l_hold0 = net.eval(ntrain, holdout) if holdout else None
+    # rotated copies go after the held-out samples; training draws from [0, ntrain) and the copies
+    rng = np.random.default_rng(seed)
+    rots = rotations()[1:]
+    for x, t in samples[:ntrain]:
+        for r in rng.choice(len(rots), size=augment, replace=False):
+            net.add_sample(*rotate_sample(x, t, Nl, rots[r]))
+    nall = net.samples
     t0 = time.perf_counter()
     hist = []
     chunk = 50
@@ -126,13 +169,21 @@ def train(nsg, samples, Nl, steps, batch, lr, holdout, seed=1):
     while done < steps:
         k = min(chunk, steps - done)
         rate = lr * (0.1 if done >= 0.8 * steps else 1.0)
-        hist.extend(net.train(k, batch=batch, lr=rate, seed=seed + done, first=0, count=ntrain).tolist())
+        if augment:
+            # [0, ntrain) and the rotated copies [ntrain + holdout, nall): alternate the two ranges
+            # in proportion to their sizes
+            n_aug = nall - ntrain - holdout
+            use_aug = ((done // chunk) % (augment + 1)) != 0
+            first, count = (ntrain + holdout, n_aug) if use_aug else (0, ntrain)
+        else:
+            first, count = 0, ntrain
+        hist.extend(net.train(k, batch=batch, lr=rate, seed=seed + done, first=first, count=count).tolist())
         done += k
     t_train = time.perf_counter() - t0
     res = {"sigma_in": sig_in, "sigma_out": sig_out, "train_samples": ntrain, "holdout_samples": holdout,
            "loss_train_initial": l_train0, "loss_train_final": net.eval(0, ntrain),
            "loss_holdout_initial": l_hold0, "loss_holdout_final": net.eval(ntrain, holdout) if holdout else None,
-           "steps": steps, "batch": batch, "lr": lr, "train_s": t_train,
+           "steps": steps, "batch": batch, "lr": lr, "train_s": t_train, "rotated_copies_per_sample": augment,
            "loss_history_every_50": hist[::50]}
     # SGS dissipation of the model against the truth on the training samples; the gain that
     # matches them calibrates the output (a-priori, training data only)
@@ -201,10 +252,11 @@ def main():
     ap.add_argument("--spin-n", type=int, default=256, help="grid of the spin-up run")
     ap.add_argument("--spin-time", type=float, default=20.0, help="spin-up length in time units")
     ap.add_argument("--settle", type=int, default=1500, help="steps on the DNS grid before sampling")
-    ap.add_argument("--samples", type=int, default=64)
-    ap.add_argument("--every", type=int, default=60)
+    ap.add_argument("--samples", type=int, default=96)
+    ap.add_argument("--every", type=int, default=80)
     ap.add_argument("--holdout", type=int, default=8)
-    ap.add_argument("--train-steps", type=int, default=1500)
+    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--augment", type=int, default=3, help="rotated copies of each training sample")
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--les-steps", type=int, default=5000)
@@ -215,7 +267,7 @@ def main():
     samples, ek_ref, ub, dstats = generate(nsg, a.dns, a.les, kc, a.nu, a.spin_n, a.spin_time, a.settle, a.samples,
                                            a.every)
     log(f"[les] {len(samples)} samples; DNS {dstats['dns_ms_per_step']:.2f} ms/step")
-    net, tstats = train(nsg, samples, a.les, a.train_steps, a.batch, a.lr, a.holdout)
+    net, tstats = train(nsg, samples, a.les, a.train_steps, a.batch, a.lr, a.holdout, augment=a.augment)
     log(f"[les] training: loss {tstats['loss_train_initial']:.4f} -> {tstats['loss_train_final']:.4f}, holdout "
         f"{tstats['loss_holdout_initial']} -> {tstats['loss_holdout_final']}, {tstats['train_s']:.1f} s")
     # measures are volume integrals over (2 pi)^3 while eps_in is a volume average

@@ -110,3 +110,27 @@ def test_closure_term_equals_nonlinear_term(oracle):
     iso = np.zeros_like(tau)
     iso[:, 0] = iso[:, 1] = iso[:, 2] = rng.standard_normal(N ** 3)
     assert np.max(np.abs(C.closure_term(iso, N))) < 1e-10
+
+
+def test_rotated_samples_match_rotated_dns(oracle):
+    """the training augmentation (3d_navier_stokes_amd/les.py rotate_sample) equals the subgrid
+    sample of the rotated DNS field, for several of the 24 cube rotations"""
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("les_pipeline", os.path.join(ROOT, "3d_navier_stokes_amd", "les.py"))
+    L = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(L)
+    rots = L.rotations()
+    assert len(rots) == 24 and all(round(np.linalg.det(R)) == 1 for R in rots)
+    N, Nl, kc = 24, 12, 4.0
+    uh = oracle.ic_k4(N, 3.0, 0.5, 5)
+    u = C.c2r(uh, N)
+    _, x, tau = C.sgs_sample(uh, N, Nl, kc)
+    q = np.indices((N, N, N)).reshape(3, -1)
+    for R in rots[1:24:5]:
+        xr, tr = L.rotate_sample(x, tau, Nl, R)
+        p = (R.T @ q) % N
+        ur = (u.reshape(-1, 3)[(p[0] * N + p[1]) * N + p[2]] @ R.T).reshape(N, N, N, 3)
+        _, x2, t2 = C.sgs_sample(C.r2c(ur), N, Nl, kc)
+        assert np.abs(xr - x2).max() < 1e-6 * np.abs(x2).max()
+        assert np.abs(tr - t2).max() < 1e-6 * np.abs(t2).max()
