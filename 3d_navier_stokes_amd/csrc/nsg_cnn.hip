@@ -87,20 +87,31 @@ struct ConvArgs {
     int N;
 };
 
-// grid (tiles, 1, batch); block 512
+// grid (tiles, 1, batch); block 512.  The layer's weights are staged in LDS next to the input
+// halo (row stride 27 CIN + 8 halfwords: the 16 rows of one B fragment fall in distinct banks);
+// the next K step's fragments are loaded while the current step's MFMAs run; bf16 outputs leave
+// through an LDS tile as 16-byte stores.
 template <int CIN, int COUT, int EPI>
 __global__ void __launch_bounds__(CONV_THREADS) conv3_kernel(ConvArgs a) {
     constexpr int SP = CIN + 8;
     constexpr int KT = 27 * CIN;
+    constexpr int WS = KT + 8;
     constexpr int KSTEPS = (KT + 31) / 32;
     constexpr int NH = COUT / 16;
     constexpr int SUB = TPTS / 16 / (CONV_THREADS / 64);  // z-rows per wavefront
-    __shared__ __attribute__((aligned(16))) uint16_t lds[HPTS * SP];
+    constexpr int OS = COUT + 8;                          // output tile point stride (halfwords)
+    static_assert(TPTS * OS <= HPTS * SP, "output tile must fit in the halo buffer");
+    __shared__ __attribute__((aligned(16))) uint16_t lds[HPTS * SP + COUT * WS];
+    uint16_t* ldw = lds + HPTS * SP;
     const int N = a.N;
     const long long N3 = (long long)N * N * N;
     int x0, y0, z0;
     tile_origin(blockIdx.x, N, x0, y0, z0);
     const int b = blockIdx.z;
+    for (int i = threadIdx.x; i < COUT * (KT / 8); i += CONV_THREADS) {
+        const int r = i / (KT / 8), c = i % (KT / 8);
+        *(u32x4*)(ldw + r * WS + c * 8) = *(const u32x4*)(a.w + (long long)r * KT + c * 8);
+    }
     stage_halo<CIN, SP>(lds, a.in + (long long)b * N3 * CIN, N, x0, y0, z0);
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -110,31 +121,43 @@ __global__ void __launch_bounds__(CONV_THREADS) conv3_kernel(ConvArgs a) {
     for (int s = 0; s < SUB; ++s)
 #pragma unroll
         for (int h = 0; h < NH; ++h) acc[s][h] = floatx4{0.f, 0.f, 0.f, 0.f};
-    for (int ks = 0; ks < KSTEPS; ++ks) {
+    auto load = [&](int ks, bf16x8 (&af)[SUB], bf16x8 (&bw)[NH]) {
         const int k0 = 32 * ks + 8 * g;
         const int tap = k0 / CIN, ch0 = k0 % CIN;
-        bf16x8 af[SUB], bw[NH];
         if (tap < 27) {
             const int dx = tap / 9, dy = (tap / 3) % 3, dz = tap % 3;
 #pragma unroll
-            for (int h = 0; h < NH; ++h) bw[h] = ld16(a.w + (long long)(h * 16 + m) * KT + k0);
+            for (int h = 0; h < NH; ++h) bw[h] = ld16(ldw + (h * 16 + m) * WS + k0);
 #pragma unroll
             for (int s = 0; s < SUB; ++s) {
                 const int st = wv * SUB + s, ix = st / TYT, iy = st % TYT;
                 af[s] = ld16(lds + (((ix + dx) * HY + (iy + dy)) * HZ + (m + dz)) * SP + ch0);
             }
-        } else {  // K padding of CIN = 16 (27 taps = 13.5 K-steps)
+        } else {  // K padding of CIN = 16 (27 taps = 13.5 K steps)
 #pragma unroll
             for (int h = 0; h < NH; ++h) bw[h] = bf16x8{};
 #pragma unroll
             for (int s = 0; s < SUB; ++s) af[s] = bf16x8{};
         }
+    };
+    bf16x8 af[SUB], bw[NH];
+    load(0, af, bw);
+    for (int ks = 0; ks < KSTEPS; ++ks) {
+        bf16x8 an[SUB], bn[NH];
+        if (ks + 1 < KSTEPS) load(ks + 1, an, bn);
 #pragma unroll
         for (int s = 0; s < SUB; ++s)
 #pragma unroll
             for (int h = 0; h < NH; ++h) acc[s][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[s], bw[h], acc[s][h], 0, 0, 0);
+        if (ks + 1 < KSTEPS) {
+#pragma unroll
+            for (int s = 0; s < SUB; ++s) af[s] = an[s];
+#pragma unroll
+            for (int h = 0; h < NH; ++h) bw[h] = bn[h];
+        }
     }
     // epilogue: lane holds rows (points) 4g..4g+3 of column (channel) m of each block
+    if constexpr (EPI != EPI_LINEAR) __syncthreads();  // the halo buffer becomes the output tile
 #pragma unroll
     for (int s = 0; s < SUB; ++s) {
         const int st = wv * SUB + s, ix = st / TYT, iy = st % TYT;
@@ -146,18 +169,29 @@ __global__ void __launch_bounds__(CONV_THREADS) conv3_kernel(ConvArgs a) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const long long idx = (row + 4 * g + i) * COUT + co;
+                const int tp = st * TZT + 4 * g + i;  // point index within the tile (z fastest)
                 const float v = acc[s][h][i] + bias;
                 if constexpr (EPI == EPI_GELU) {
-                    a.act_out[idx] = f2bf(gelu(v));
+                    lds[tp * OS + co] = f2bf(gelu(v));
                 } else if constexpr (EPI == EPI_GELU_PRE) {
                     a.f_out[idx] = v;
-                    a.act_out[idx] = f2bf(gelu(v));
+                    lds[tp * OS + co] = f2bf(gelu(v));
                 } else if constexpr (EPI == EPI_LINEAR) {
                     a.f_out[idx] = v;
                 } else {
-                    a.act_out[idx] = f2bf(v * gelu_grad(a.pre_in[idx]));
+                    lds[tp * OS + co] = f2bf(v * gelu_grad(a.pre_in[idx]));
                 }
             }
+        }
+    }
+    if constexpr (EPI != EPI_LINEAR) {
+        __syncthreads();
+        constexpr int CH = COUT / 8;
+        for (int i = threadIdx.x; i < TPTS * CH; i += CONV_THREADS) {
+            const int tp = i / CH, c = i % CH;
+            const int iz = tp % TZT, iy = (tp / TZT) % TYT, ix = tp / (TZT * TYT);
+            const long long gp = (long long)b * N3 + ((long long)(x0 + ix) * N + (y0 + iy)) * N + (z0 + iz);
+            *(u32x4*)(a.act_out + gp * COUT + c * 8) = *(const u32x4*)(lds + tp * OS + c * 8);
         }
     }
 }
