@@ -246,10 +246,15 @@ def main():
     s32.step(1)
     m64, ek64, _ = s64.measures()
     m32, ek32, _ = s32.measures()
-    sel = ek64 > 1e-300
+    # shells holding at least 1e-10 of the peak shell's energy: below that, FP32 round-off of the
+    # transforms is an absolute noise floor and a relative error measures the floor
+    sel = ek64 >= 1e-10 * ek64.max()
     ek_err = float(np.max(np.abs(ek32[sel] - ek64[sel]) / ek64[sel]))
     kc = N // 3
-    ek_err_kc = float(np.max(np.abs(ek32[1:kc + 1] - ek64[1:kc + 1]) / ek64[1:kc + 1]))
+    k = np.arange(1, kc + 1)
+    selk = k[sel[1:kc + 1]]
+    ek_err_kc = float(np.max(np.abs(ek32[selk] - ek64[selk]) / ek64[selk]))
+    n_resolved = int(np.count_nonzero(sel))
     u64 = s64.get_state()
     u32 = s32.get_state()
     state_err = float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64))
@@ -285,7 +290,8 @@ def main():
                  "roofline": r64["roofline"], "passes": r64["passes"], "step_bytes_GB": r64["step_bytes_GB"],
                  "step_GBps": r64["step_GBps"]},
         "fp32bulk_over_fp64": round(r32["steps_per_s"] / r64["steps_per_s"], 3),
-        "fp32bulk_error_vs_fp64_one_step": {"E_k_max_rel": ek_err, "E_k_max_rel_k_le_kc": ek_err_kc,
+        "fp32bulk_error_vs_fp64_one_step": {"E_k_max_rel_resolved_shells": ek_err, "E_k_max_rel_k_le_kc": ek_err_kc,
+                                            "resolved_shells": n_resolved,
                                             "energy_rel": abs(m32.energy - m64.energy) / m64.energy,
                                             "dissipation_rel": abs(m32.dissipation - m64.dissipation) / m64.dissipation,
                                             "state_rel_l2": state_err},
