@@ -74,6 +74,7 @@ template <typename T, int N, int S> static constexpr int WYof() { return is64<T>
 // elements per thread spills under its 128-VGPR cap; 512 threads of 16 elements do not (measured
 // 0.51 -> 0.48 ms per launch)
 template <typename T, int N, int S> static constexpr int E5of() { return (!is64<T>() && N == 512) ? 16 : EXof<T, N, S>(); }
+template <typename T, int N, int S> static constexpr int W5of() { return WXof<T, N, S>(); }
 
 template <typename F> static int dispatch_N(int N, F&& f) {
     switch (N) {
@@ -267,7 +268,7 @@ static int launch_passz(nsg_ctx* c, const void* in, void* out, long long ibox, l
 
 template <typename T, int N, int S, int MODE>
 static int launch_pass5(nsg_ctx* c, double2* uh, int b0, int nb, int stage, void* kout, int force, double norm) {
-    constexpr int EPT = E5of<T, N, S>(), W = WXof<T, N, S>(), TPF = N / EPT;
+    constexpr int EPT = E5of<T, N, S>(), W = W5of<T, N, S>(), TPF = N / EPT;
     static_assert(TPF * W <= 1024, "block too large");
     constexpr size_t e = esz<T>();
     P5Args a;

@@ -63,6 +63,8 @@ def generate(nsg, Nd, Nl, kc, nu, spin_n, spin_time, settle, nsamples, every, se
     while spin.time()[0] < spin_time:
         spin.step(100)
         nspin += 100
+        if nspin % 2000 == 0:
+            log(f"[les] spin-up {nspin} steps, t = {spin.time()[0]:.3f}")
     dns = forced(nsg, Nd, nu, precision)
     spin.resample_to(dns)
     spin.close()
@@ -83,6 +85,8 @@ def generate(nsg, Nd, Nl, kc, nu, spin_n, spin_time, settle, nsamples, every, se
         spectra.append(ek)
         enst.append(tot.enstrophy)
         energy.append(dns.measures()[0].energy)
+        if (s + 1) % 16 == 0:
+            log(f"[les] sample {s + 1}/{nsamples}, t = {dns.time()[0]:.3f}")
     t_phys = dns.time()[0] - t_phys0
     ub = les.get_state()
     h = len(spectra) // 2
@@ -158,9 +162,11 @@ def train(nsg, samples, Nl, steps, batch, lr, holdout, seed=1, augment=0):
     # rotated copies go after the held-out samples; training draws from [0, ntrain) and the copies
     rng = np.random.default_rng(seed)
     rots = rotations()[1:]
-    for x, t in samples[:ntrain]:
+    for i, (x, t) in enumerate(samples[:ntrain]):
         for r in rng.choice(len(rots), size=augment, replace=False):
             net.add_sample(*rotate_sample(x, t, Nl, rots[r]))
+        if augment and (i + 1) % 32 == 0:
+            log(f"[les] rotated copies of {i + 1}/{ntrain} training samples added")
     nall = net.samples
     t0 = time.perf_counter()
     hist = []
@@ -179,6 +185,8 @@ def train(nsg, samples, Nl, steps, batch, lr, holdout, seed=1, augment=0):
             first, count = 0, ntrain
         hist.extend(net.train(k, batch=batch, lr=rate, seed=seed + done, first=first, count=count).tolist())
         done += k
+        if done % 1000 == 0:
+            log(f"[les] training step {done}/{steps}, loss {np.mean(hist[-50:]):.4f}")
     t_train = time.perf_counter() - t0
     res = {"sigma_in": sig_in, "sigma_out": sig_out, "train_samples": ntrain, "holdout_samples": holdout,
            "loss_train_initial": l_train0, "loss_train_final": net.eval(0, ntrain),
