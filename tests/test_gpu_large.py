@@ -81,3 +81,26 @@ def test_determinism_512(nsg):
     assert np.array_equal(out[0][0], out[1][0])
     assert out[0][1] == out[1][1]
     assert np.array_equal(out[0][2], out[1][2]) and np.array_equal(out[0][3], out[1][3])
+
+
+def test_capacity_1024_fp64(nsg):
+    """the 1024^3 FP64 capacity run (about 98 GB of device memory): the corrected Shapiro field is
+    Beltrami (nonlinear term zero), so its energy decays by exactly ((2-D)/(2+D))^(2n) under RK4-CN
+    and E, Ens, Hel keep the ratios of one shell |k|^2 = 12; a k^4 field with nu = 0 keeps its
+    energy (sum Re(u* . NL) = 0) to the RK4 truncation error"""
+    N, nu, dt = 1024, 1e-3, 1e-3
+    with nsg.Solver(N, nu=nu, dt=dt) as s:
+        s.init("shapiro")
+        t0 = s.measures()[0]
+        s.step(2)
+        t1 = s.measures()[0]
+        D = nu * 12 * dt
+        assert abs(t1.energy / t0.energy - ((2 - D) / (2 + D)) ** 4) < 1e-12
+        assert abs(t1.enstrophy / t1.energy - 12.0) < 1e-10
+        assert abs(t1.helicity / t1.energy - 12 ** 0.5) < 1e-10
+    with nsg.Solver(N, nu=0.0, dt=1e-4) as s:
+        s.init("k4", k0=4.0, energy=0.5, seed=11)
+        e0 = s.measures()[0].energy
+        s.step(2)
+        e1 = s.measures()[0].energy
+        assert np.isfinite(e1) and abs(e1 - e0) < 1e-9 * e0
