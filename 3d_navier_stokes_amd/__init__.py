@@ -25,13 +25,13 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libnsg.so")
 
-SCHEMES = {"rk4cn": 0, "ifrk4": 1}
+SCHEMES = {"rk4cn": 0, "ifrk4": 1, "rk5cn": 2, "dp5": 3}
 PRECISIONS = {"fp64": 0, "fp32bulk": 1}
 ICS = {"taylor_green": 0, "shapiro": 1, "k4": 2}
 SUPPORTED_N = (8, 12, 16, 24, 32, 48, 64, 96, 128, 256, 512, 1024)
 
 ERRORS = {-1: "NSG_ERR_INVALID", -2: "NSG_ERR_OOM", -3: "NSG_ERR_DEVICE", -4: "NSG_ERR_NONFINITE",
-          -5: "NSG_ERR_BLOWUP"}
+          -5: "NSG_ERR_BLOWUP", -6: "NSG_ERR_STEP"}
 
 
 class NSGError(RuntimeError):
@@ -77,6 +77,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.nsg_init_state.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_uint64]
     lib.nsg_rhs.argtypes = [vp, vp, vp]
     lib.nsg_step.argtypes = [vp, ctypes.c_long]
+    lib.nsg_dp_stats.argtypes = [vp, P(ctypes.c_longlong), P(ctypes.c_longlong), dp]
     lib.nsg_measures.argtypes = [vp, P(_Totals), vp, vp, ctypes.c_int, ctypes.c_int]
     lib.nsg_realspace.argtypes = [vp, ctypes.c_int, vp]
     lib.nsg_get_time.argtypes = [vp, dp, dp, dp]
@@ -248,7 +249,14 @@ class Solver:
         return out
 
     def step(self, nsteps: int = 1):
+        """nsteps time steps (scheme "dp5": nsteps accepted adaptive steps)"""
         self._check(self.lib.nsg_step(self._h, int(nsteps)))
+
+    def dp_stats(self):
+        """scheme "dp5": (accepted steps, rejected tries, error norm of the last try)"""
+        a, r, e = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_double()
+        self._check(self.lib.nsg_dp_stats(self._h, ctypes.byref(a), ctypes.byref(r), ctypes.byref(e)))
+        return a.value, r.value, e.value
 
     def measures(self, quirk: bool = False):
         t = _Totals()

@@ -176,6 +176,28 @@ template <typename T, int W> static dim3 xgrid(int ntiles, int nrows, int nb) {
     return dim3((unsigned)(16 * ((pairs + 7) / 8)), 1, nb);
 }
 
+// Dormand-Prince 5(4) tableau, Solver/src/solver.c:34-40; row s-1 forms the input of stage s.
+// A32 is the published 9/40: the reference's 0.5 breaks sum_j A_3j = C3 and makes the scheme
+// first order.
+static const double kDPA[7][MAXSLOPES] = {
+    {0, 0, 0, 0, 0, 0},
+    {1.0 / 5.0, 0, 0, 0, 0, 0},
+    {3.0 / 40.0, 9.0 / 40.0, 0, 0, 0, 0},
+    {44.0 / 45.0, -56.0 / 15.0, 32.0 / 9.0, 0, 0, 0},
+    {19372.0 / 6561.0, -25360.0 / 2187.0, 64448.0 / 6561.0, -212.0 / 729.0, 0, 0},
+    {9017.0 / 3168.0, -355.0 / 33.0, 46732.0 / 5247.0, 49.0 / 176.0, -5103.0 / 18656.0, 0},
+    {35.0 / 384.0, 0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0}};
+
+static bool dp_family(const nsg_ctx* c) { return c->p.scheme == NSG_SCHEME_RK5CN || c->p.scheme == NSG_SCHEME_DP5; }
+
+// slopes of boxes from b0 and the tableau row of `stage` (zero row outside the Dormand-Prince family)
+template <typename T> static void fill_row(const nsg_ctx* c, StageRow& r, int b0, int stage) {
+    for (int j = 0; j < MAXSLOPES; ++j) {
+        r.k[j] = c->K[j] ? (const char*)c->K[j] + b0 * c->sbox * esz<T>() : nullptr;
+        r.a[j] = (dp_family(c) && stage >= 1 && stage <= 7 && j < c->nslopes) ? kDPA[stage - 1][j] : 0.0;
+    }
+}
+
 template <typename T, int N, int S>
 static int launch_pass1(nsg_ctx* c, const double2* uh, int b0, int nb, const void* kprev, int stage, int sel, int force) {
     constexpr int EPT = EXof<T, N, S>(), W = WXof<T, N, S>(), TPF = N / EPT;
@@ -183,6 +205,7 @@ static int launch_pass1(nsg_ctx* c, const double2* uh, int b0, int nb, const voi
     P1Args a;
     a.uh = uh + b0 * c->sbox;
     a.kprev = kprev ? (const char*)kprev + b0 * c->sbox * esz<T>() : nullptr;
+    fill_row<T>(c, a.row, b0, stage);
     a.out = (char*)c->B1 + b0 * c->a1box * esz<T>();
     a.twN = twN_of<T>(c); a.sc = c->sc + b0; a.fslots = c->fslots + (long long)b0 * c->NK;
     a.kf2 = c->p.force_kf * c->p.force_kf;
@@ -191,7 +214,10 @@ static int launch_pass1(nsg_ctx* c, const double2* uh, int b0, int nb, const voi
     a.kmax = c->kmax; a.NK = c->NK; a.NZP = c->NZP;
     const dim3 grid = xgrid<T, W>(c->NZP / W, c->NK, nb);
     prof_begin(c);
-    hipLaunchKernelGGL((pass1_kernel<T, N, EPT, W>), grid, dim3(TPF * W), 0, c->s, a);
+    if (dp_family(c))
+        hipLaunchKernelGGL((pass1_kernel<T, N, EPT, W, 1>), grid, dim3(TPF * W), 0, c->s, a);
+    else
+        hipLaunchKernelGGL((pass1_kernel<T, N, EPT, W, 0>), grid, dim3(TPF * W), 0, c->s, a);
     LAUNCHCHK(c);
     prof_end(c, 0);
     return NSG_OK;
@@ -267,17 +293,20 @@ static int launch_passz(nsg_ctx* c, const void* in, void* out, long long ibox, l
 }
 
 template <typename T, int N, int S, int MODE>
-static int launch_pass5(nsg_ctx* c, double2* uh, int b0, int nb, int stage, void* kout, int force, double norm) {
+static int launch_pass5(nsg_ctx* c, double2* uh, int b0, int nb, int stage, void* kout, int force, double norm,
+                        double2* unew = nullptr, double* errpart = nullptr) {
     constexpr int EPT = E5of<T, N, S>(), W = W5of<T, N, S>(), TPF = N / EPT;
     static_assert(TPF * W <= 1024, "block too large");
     constexpr size_t e = esz<T>();
     P5Args a;
     a.in = (const char*)c->B2 + b0 * c->a4box * e;
     a.uh = uh + b0 * c->sbox;
-    a.k1 = (const char*)c->K[0] + b0 * c->sbox * e;
-    a.k2 = (const char*)c->K[1] + b0 * c->sbox * e;
-    a.k3 = (const char*)c->K[2] + b0 * c->sbox * e;
+    a.unew = unew ? unew + b0 * c->sbox : nullptr;
+    fill_row<T>(c, a.row, b0, stage);
     a.kout = kout ? (char*)kout + b0 * c->sbox * e : nullptr;
+    a.nslots = c->NK * (c->NZP / W);
+    a.errpart = errpart ? errpart + (long long)b0 * a.nslots * 3 : nullptr;
+    a.nstage = c->nstage;
     a.extra = (MODE == EP_STAGE && c->closure && c->cnn) ? c->closure + b0 * c->sbox : nullptr;
     a.twN = twN_of<T>(c); a.sc = c->sc + b0; a.fslots = c->fslots + (long long)b0 * c->NK;
     a.kf2 = c->p.force_kf * c->p.force_kf; a.eps_in = c->p.force_eps; a.norm = norm;
@@ -286,7 +315,14 @@ static int launch_pass5(nsg_ctx* c, double2* uh, int b0, int nb, int stage, void
     a.kmax = c->kmax; a.NK = c->NK; a.NZP = c->NZP;
     const dim3 grid = xgrid<T, W>(c->NZP / W, c->NK, nb);
     prof_begin(c);
-    hipLaunchKernelGGL((pass5_kernel<T, N, EPT, W, MODE>), grid, dim3(TPF * W), 0, c->s, a);
+    if constexpr (MODE == EP_STAGE) {
+        if (dp_family(c))
+            hipLaunchKernelGGL((pass5_kernel<T, N, EPT, W, MODE, 1>), grid, dim3(TPF * W), 0, c->s, a);
+        else
+            hipLaunchKernelGGL((pass5_kernel<T, N, EPT, W, MODE, 0>), grid, dim3(TPF * W), 0, c->s, a);
+    } else {
+        hipLaunchKernelGGL((pass5_kernel<T, N, EPT, W, MODE, 0>), grid, dim3(TPF * W), 0, c->s, a);
+    }
     LAUNCHCHK(c);
     prof_end(c, 4);
     return NSG_OK;
@@ -315,18 +351,29 @@ static int run_stage(nsg_ctx* c, double2* uh, int b0, int nb, int stage, bool tr
         LAUNCHCHK(c);
     }
     if ((rc = launch_passy<T, N, S, -1>(c, b0, nb, 3))) return rc;
-    const int st = final_update ? 4 : (stage < 4 ? stage : 1);
-    void* kout = (st < 4) ? c->K[st - 1] : nullptr;
-    return launch_pass5<T, N, S, EP_STAGE>(c, uh, b0, nb, st, kout, force, 1.0 / (N3 * N3));
+    void* kout = final_update ? nullptr : c->K[stage - 1];
+    const bool dp = final_update && c->p.scheme == NSG_SCHEME_DP5;
+    return launch_pass5<T, N, S, EP_STAGE>(c, uh, b0, nb, final_update ? c->nstage : stage, kout, force,
+                                           1.0 / (N3 * N3), dp ? c->ucand : nullptr, dp ? c->dperr : nullptr);
 }
 
+// one step of boxes [b0, b0+nb); Dormand-Prince: one try, whose candidate lands in ucand and
+// whose error norm and step control land in the box scalars (the caller accepts by swapping)
 template <typename T, int N, int S> static int run_step(nsg_ctx* c, int b0, int nb) {
     int rc;
     hipLaunchKernelGGL(step_begin_kernel, dim3(1), dim3(64), 0, c->s, c->sc + b0, nb);
     LAUNCHCHK(c);
-    for (int s = 1; s <= 4; ++s)
-        if ((rc = run_stage<T, N, S>(c, c->uh, b0, nb, s, s == 1, s == 4))) return rc;
-    hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(64), 0, c->s, c->sc + b0, nb);
+    for (int s = 1; s <= c->nstage; ++s)
+        if ((rc = run_stage<T, N, S>(c, c->uh, b0, nb, s, s == 1, s == c->nstage))) return rc;
+    if (c->p.scheme == NSG_SCHEME_DP5) {
+        constexpr int W = W5of<T, N, S>();
+        const int nslots = c->NK * (c->NZP / W);
+        const double count = (double)N * N * (N / 2 + 1);
+        hipLaunchKernelGGL(dp_finalize_kernel, dim3(nb), dim3(256), 0, c->s, c->sc + b0,
+                           c->dperr + (long long)b0 * nslots * 3, nslots, count);
+    } else {
+        hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(64), 0, c->s, c->sc + b0, nb);
+    }
     LAUNCHCHK(c);
     return NSG_OK;
 }
@@ -367,8 +414,8 @@ static int check_boxes(nsg_ctx* c) {
 static void free_all(nsg_ctx* c) {
     if (!c) return;
     nsgi_closure_free(c);
-    void* bufs[] = {c->uh, c->aux, c->K[0], c->K[1], c->K[2], c->B1, c->B2, c->twN64, c->twM64,
-                    c->twN32, c->twM32, c->sc, c->fslots, c->part};
+    void* bufs[] = {c->uh, c->aux, c->K[0], c->K[1], c->K[2], c->K[3], c->K[4], c->K[5], c->ucand, c->dperr,
+                    c->B1, c->B2, c->twN64, c->twM64, c->twN32, c->twM32, c->sc, c->fslots, c->part};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev[0]) (void)hipEventDestroy(c->ev[0]);
@@ -382,7 +429,12 @@ static const char* check_params(const nsg_params* p) {
     if (dispatch_N(N, [](auto) { return 0; }) != 0) return "unsupported N (8,12,16,24,32,48,64,96,128,256,512,1024)";
     if (!(p->nu >= 0.0) || !(p->dt > 0.0)) return "nu must be >= 0 and dt > 0";
     if (p->visc_pow < 1 || p->visc_pow > 4) return "visc_pow must be 1..4";
-    if (p->scheme != NSG_SCHEME_RK4CN && p->scheme != NSG_SCHEME_IFRK4) return "unknown scheme";
+    if (p->scheme < NSG_SCHEME_RK4CN || p->scheme > NSG_SCHEME_DP5) return "unknown scheme";
+    if (p->scheme == NSG_SCHEME_DP5 && p->cfl > 0.0) return "the Dormand-Prince scheme chooses dt itself: cfl must be 0";
+    // FP32 round-off of the slopes at low-amplitude modes exceeds the 1e-7 per-mode tolerance
+    // (measured: the error norm of one try reads 8% high at 32^3, 2.3x at 128^3)
+    if (p->scheme == NSG_SCHEME_DP5 && p->precision != NSG_FP64)
+        return "the Dormand-Prince error norm (tolerance 1e-7 per mode) is below FP32 round-off: use FP64";
     if (p->precision != NSG_FP64 && p->precision != NSG_FP32BULK) return "unknown precision";
     if (p->force_eps > 0.0 && !(p->force_kf > 0.0 && p->force_kf < 7.5)) return "force_kf must be in (0, 7.5)";
     return nullptr;
@@ -409,9 +461,13 @@ static int create_batch(const nsg_params* p, int nbox, nsg_ctx** out) {
         if (!same_batch(p[b], p[0]))
             return fail(c, NSG_ERR_INVALID, "boxes of one batch must share N, precision, scheme, visc_pow, forcing, device");
     }
+    if (p[0].scheme == NSG_SCHEME_DP5 && nbox != 1)
+        return fail(c, NSG_ERR_INVALID, "the adaptive Dormand-Prince scheme runs one box per context");
     const int N = p[0].N[0];
     c->N = N;
     c->nbox = nbox;
+    c->nslopes = p[0].scheme == NSG_SCHEME_DP5 ? 6 : p[0].scheme == NSG_SCHEME_RK5CN ? 5 : 3;
+    c->nstage = p[0].scheme == NSG_SCHEME_DP5 ? 7 : p[0].scheme == NSG_SCHEME_RK5CN ? 6 : 4;
     c->kmax = N / 3;
     c->NK = 2 * c->kmax + 1;
     c->NZP = nzp_of(N);
@@ -462,8 +518,13 @@ static int create_batch(const nsg_params* p, int nbox, nsg_ctx** out) {
     int rc;
     if ((rc = alloc((void**)&c->uh, nstate * sizeof(double2)))) return rc;
     if ((rc = alloc((void**)&c->aux, nstate * sizeof(double2)))) return rc;
-    for (int i = 0; i < 3; ++i)
+    for (int i = 0; i < c->nslopes; ++i)
         if ((rc = alloc(&c->K[i], nstate * ebulk))) return rc;
+    if (p[0].scheme == NSG_SCHEME_DP5) {
+        if ((rc = alloc((void**)&c->ucand, nstate * sizeof(double2)))) return rc;
+        // one slot per final-pass block; W >= 1 bounds every schedule's count
+        if ((rc = alloc((void**)&c->dperr, sizeof(double) * 3 * NK * NZP * B))) return rc;
+    }
     if ((rc = alloc(&c->B1, b1))) return rc;
     if ((rc = alloc(&c->B2, b2))) return rc;
     if ((rc = alloc((void**)&c->sc, sizeof(DevScalars) * B))) return rc;
@@ -474,7 +535,8 @@ static int create_batch(const nsg_params* p, int nbox, nsg_ctx** out) {
     if ((rc = upload_table(c, N / 2, &c->twM64, &c->twM32))) return rc;
     HIPCHK(c, hipMemset(c->uh, 0, nstate * sizeof(double2)));
     HIPCHK(c, hipMemset(c->fslots, 0, sizeof(double) * NK * B));
-    for (int i = 0; i < 3; ++i) HIPCHK(c, hipMemset(c->K[i], 0, nstate * ebulk));
+    for (int i = 0; i < c->nslopes; ++i) HIPCHK(c, hipMemset(c->K[i], 0, nstate * ebulk));
+    if (c->ucand) HIPCHK(c, hipMemset(c->ucand, 0, nstate * sizeof(double2)));
     std::vector<DevScalars> hs(B);
     for (int b = 0; b < nbox; ++b) {
         hs[b] = DevScalars{};
@@ -716,10 +778,43 @@ int nsg_rhs(nsg_ctx* c, const double* u_hat, double* out) {
     return download_compact(c, c->K[0], true, out);
 }
 
+// Dormand-Prince: tries until one is accepted (at most 50, Solver/src/data_types.h:128), each
+// followed by a read of the box scalars; an accepted candidate becomes the state by a swap
+static int dp_steps(nsg_ctx* c, long nsteps) {
+    constexpr int MAX_TRY = 50;
+    int rc;
+    for (long i = 0; i < nsteps; ++i) {
+        if (c->cnn && (rc = nsgi_closure_eval(c))) return rc;
+        int tries = 0;
+        for (;;) {
+            if ((rc = enqueue_steps(c, 1, 0, 1))) return rc;
+            DevScalars h;
+            HIPCHK(c, hipMemcpyAsync(&h, c->sc, sizeof h, hipMemcpyDeviceToHost, c->s));
+            HIPCHK(c, hipStreamSynchronize(c->s));
+            if (!std::isfinite(h.dperr) || !std::isfinite(h.dt))
+                return fail(c, NSG_ERR_NONFINITE, "non-finite Dormand-Prince error (%g) or dt (%g) at step %lld", h.dperr,
+                            h.dt, h.steps);
+            if (h.dpacc) {
+                std::swap(c->uh, c->ucand);
+                if (c->max_speed[0] > 0.0 && h.max_last > c->max_speed[0])
+                    return fail(c, NSG_ERR_BLOWUP, "max|u| = %g exceeds the limit %g at step %lld", h.max_last,
+                                c->max_speed[0], h.steps);
+                break;
+            }
+            ++c->dp_rejected;
+            if (++tries >= MAX_TRY)
+                return fail(c, NSG_ERR_STEP, "Dormand-Prince step rejected %d times (last error %g, dt now %g)", MAX_TRY,
+                            h.dperr, h.dt);
+        }
+    }
+    return NSG_OK;
+}
+
 int nsg_step(nsg_ctx* c, long nsteps) {
     if (!c || nsteps < 0) return NSG_ERR_INVALID;
     HIPCHK(c, hipSetDevice(c->p.device));
     int rc;
+    if (c->p.scheme == NSG_SCHEME_DP5) return dp_steps(c, nsteps);
     if (c->cnn) {
         // learned closure: the model term is evaluated from the state at the start of each step
         // and held fixed through its four stages
@@ -735,6 +830,19 @@ int nsg_step(nsg_ctx* c, long nsteps) {
         return NSG_OK;
     }
     return check_boxes(c);
+}
+
+int nsg_dp_stats(nsg_ctx* c, long long* accepted, long long* rejected, double* last_err) {
+    if (!c) return NSG_ERR_INVALID;
+    if (c->p.scheme != NSG_SCHEME_DP5) return fail(c, NSG_ERR_INVALID, "not a Dormand-Prince context");
+    HIPCHK(c, hipSetDevice(c->p.device));
+    DevScalars h;
+    HIPCHK(c, hipMemcpyAsync(&h, c->sc, sizeof h, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    if (accepted) *accepted = h.steps;
+    if (rejected) *rejected = c->dp_rejected;
+    if (last_err) *last_err = h.dperr;
+    return NSG_OK;
 }
 
 int nsg_measures(nsg_ctx* c, nsg_totals* out, double* E_k, double* Ens_k, int n_spect, int quirk) {
@@ -765,20 +873,31 @@ static void pass_bytes(const nsg_ctx* c, double* b /*[5]: stage-averaged, one bo
     const double N = c->N, NK = c->NK, NZL = c->kmax + 1;
     const double E = c->fp32 ? 8.0 : 16.0, S = 16.0;
     const double st = 3 * NK * NK * NZL;  // complex elements of a 3-vector state
-    // pass 1: read state (+ K_{s-1} for s > 1), write 6 fields
-    b[0] = st * S + 0.75 * st * E + 6 * N * NK * NZL * E;
+    const double ns = c->nstage;
+    // slopes read by the stage inputs, summed over the step: one per stage s > 1 (RK4 schemes), the
+    // nonzero entries of the tableau rows (Dormand-Prince)
+    double kin = ns - 1;
+    if (dp_family(c)) {
+        kin = 0;
+        for (int s = 2; s <= c->nstage; ++s)
+            for (int j = 0; j < s - 1; ++j) kin += kDPA[s - 1][j] != 0.0;
+    }
+    // slopes read by the final update: K1..K3 (RK4), K1,K3,K4,K5 (+K6 for Dormand-Prince)
+    const double kfin = dp_family(c) ? (c->nstage == 7 ? 5 : 4) : 3;
+    // pass 1: read state (+ slopes for s > 1), write 6 fields
+    b[0] = st * S + kin / ns * st * E + 6 * N * NK * NZL * E;
     b[1] = (6 * N * NK * NZL + 6 * N * N * NZL) * E;
     b[2] = (6 * N * N * NZL + 3 * N * N * NZL) * E;
     b[3] = (3 * N * N * NZL + 3 * N * NK * NZL) * E;
-    // pass 5: read A4, write K_s (s<4) | read state + K1..K3, write state (s=4)
-    b[4] = 3 * N * NK * NZL * E + 0.75 * st * E + 0.25 * (st * S + 3 * st * E + st * S);
+    // pass 5: read A4, write K_s (s < nstage) | read state + slopes, write state (final stage)
+    b[4] = 3 * N * NK * NZL * E + (ns - 1) / ns * st * E + (st * S + kfin * st * E + st * S) / ns;
 }
 
 double nsg_step_bytes(const nsg_ctx* c) {
     if (!c) return 0.0;
     double b[5];
     pass_bytes(c, b);
-    return 4.0 * (b[0] + b[1] + b[2] + b[3] + b[4]) * c->nbox;
+    return c->nstage * (b[0] + b[1] + b[2] + b[3] + b[4]) * c->nbox;
 }
 
 int nsg_profile_passes(nsg_ctx* c, int iters, double* ms, double* bytes) {
@@ -915,6 +1034,9 @@ int nsg_ensemble_create(const nsg_params* boxes, int nbox, nsg_ensemble** out) {
     e->where.resize(nbox);
     for (int b = 0; b < nbox; ++b) {
         if (const char* m = check_params(&boxes[b])) return efail(e, NSG_ERR_INVALID, "box " + std::to_string(b) + ": " + m);
+        if (boxes[b].scheme == NSG_SCHEME_DP5)
+            return efail(e, NSG_ERR_INVALID, "box " + std::to_string(b) +
+                                                 ": the adaptive Dormand-Prince scheme runs one box per context (nsg_create)");
         if (boxes[b].device != e->device) return efail(e, NSG_ERR_INVALID, "all boxes of an ensemble must use one device");
         int gi = -1;
         for (size_t g = 0; g < members.size(); ++g)

@@ -26,7 +26,13 @@ struct nsg_ctx {
     long long sbox = 0, a1box = 0, a2box = 0, a3box = 0, a4box = 0;
     double2* uh = nullptr;   // states [nbox][3][NK][NK][NZP]
     double2* aux = nullptr;  // a second FP64 state per box (nsg_rhs input, autotuner backup)
-    void* K[3] = {nullptr, nullptr, nullptr};
+    void* K[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // stored stage slopes
+    int nslopes = 3;                  // K buffers in use: 3 (RK4 schemes), 5 (RK5-CN), 6 (Dormand-Prince)
+    int nstage = 4;                   // RHS evaluations per step (try)
+    double2* ucand = nullptr;         // Dormand-Prince candidate state
+    double* dperr = nullptr;          // Dormand-Prince per-block error sums
+    int dp_nslots = 0;                // blocks of the final pass 5 (one error slot each)
+    long long dp_rejected = 0;        // Dormand-Prince tries rejected so far
     void* B1 = nullptr;  // A1 / A3 of every box, or one box's real AoS field
     void* B2 = nullptr;  // A2 / A4 of every box, or one box's AoS staging
     double2 *twN64 = nullptr, *twM64 = nullptr;

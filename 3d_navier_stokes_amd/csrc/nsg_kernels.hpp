@@ -27,6 +27,8 @@ struct DevScalars {
     double cfl;                  // > 0: adaptive dt
     double dx;                   // 2 pi / N
     double nu;                   // viscosity of this box
+    double dperr;                // Dormand-Prince: error norm of the last try
+    long long dpacc;             // Dormand-Prince: 1 if the last try was accepted
 };
 
 // ------------------------------------------------------------------------------------------
@@ -62,7 +64,37 @@ __device__ __forceinline__ double visc_rate(double nu, int p, long long k2) {
     return nu * r;
 }
 
-enum Scheme { RK4CN = 0, IFRK4 = 1 };
+template <typename T> __device__ __forceinline__ double2 to_d2(cpx_t<T> v) { return make_double2((double)v.x, (double)v.y); }
+template <typename T> __device__ __forceinline__ cpx_t<T> from_d2(double2 v) { return cmk<cpx_t<T>>((T)v.x, (T)v.y); }
+
+enum Scheme { RK4CN = 0, IFRK4 = 1, RK5CN = 2, DP5 = 3 };
+
+// Dormand-Prince family: stage s input u + dt sum_{j<s} a_j K_j over the stored slopes
+// (Solver/src/solver.c:236-334); the row's coefficients come from the host's tableau
+constexpr int MAXSLOPES = 6;
+struct StageRow {
+    const void* k[MAXSLOPES];  // K_1..K_6 of the first box (bulk precision)
+    double a[MAXSLOPES];       // row of the tableau (0: slope not read)
+};
+
+// ud += dt sum_j a_j K_j for j < nterm (uniform branches; fully unrolled so nothing is indexed at
+// run time)
+template <typename T>
+__device__ __forceinline__ void add_slopes(double2 (&ud)[3], const StageRow& r, int nterm, long long boxoff,
+                                           long long idx, long long plane, double dt) {
+#pragma unroll
+    for (int j = 0; j < MAXSLOPES; ++j) {
+        if (j >= nterm || r.a[j] == 0.0) continue;
+        const double cj = r.a[j] * dt;
+        const cpx_t<T>* kj = (const cpx_t<T>*)r.k[j] + boxoff;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double2 kv = to_d2<T>(kj[c * plane + idx]);
+            ud[c].x += cj * kv.x;
+            ud[c].y += cj * kv.y;
+        }
+    }
+}
 
 // stage input u~_s = alpha u_hat + beta K_{s-1}
 __device__ __forceinline__ void stage_in_coef(int scheme, int stage, double D, double dt, double& a, double& b) {
@@ -78,8 +110,6 @@ __device__ __forceinline__ void stage_in_coef(int scheme, int stage, double D, d
     }
 }
 
-template <typename T> __device__ __forceinline__ double2 to_d2(cpx_t<T> v) { return make_double2((double)v.x, (double)v.y); }
-template <typename T> __device__ __forceinline__ cpx_t<T> from_d2(double2 v) { return cmk<cpx_t<T>>((T)v.x, (T)v.y); }
 
 // max that propagates NaN (a NaN state must reach the host's non-finite check)
 __device__ __forceinline__ double nanmax(double a, double b) { return (a >= b || a != a) ? a : b; }
@@ -132,7 +162,8 @@ __device__ __forceinline__ bool tile_of_block(int ntiles, int nrows, int& tile, 
 // batches: box = blockIdx.z; per-box element strides sbox (state/K) and obox (A1)
 struct P1Args {
     const double2* uh;
-    const void* kprev;
+    const void* kprev;  // RK4 schemes: K_{s-1}
+    StageRow row;       // Dormand-Prince schemes: the stage's slopes and tableau row
     void* out;
     const void* twN;
     const DevScalars* sc;
@@ -143,7 +174,9 @@ struct P1Args {
     int kmax, NK, NZP;
 };
 
-template <typename T, int N, int EPT, int W>
+// FAM 0: RK4 schemes; FAM 1: Dormand-Prince family (separate code objects, so the RK4 kernels
+// keep their register allocation)
+template <typename T, int N, int EPT, int W, int FAM>
 __global__ void __launch_bounds__(N / EPT * W) pass1_kernel(P1Args a) {
     using C = cpx_t<T>;
     constexpr int TPF = N / EPT;
@@ -176,7 +209,9 @@ __global__ void __launch_bounds__(N / EPT * W) pass1_kernel(P1Args a) {
         double2 ud[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) ud[c] = ok ? uh[c * plane + idx] : make_double2(0.0, 0.0);
-        if (a.stage > 1 && ok) {
+        if constexpr (FAM == 1) {
+            if (a.stage > 1 && ok) add_slopes<T>(ud, a.row, a.stage - 1, box * a.sbox, idx, plane, dt);
+        } else if (a.stage > 1 && ok) {
             double al, be;
             stage_in_coef(a.scheme, a.stage, visc_rate(nu, a.visc_pow, k2), dt, al, be);
 #pragma unroll
@@ -658,22 +693,30 @@ enum EpMode { EP_STAGE = 0, EP_STORE = 1 };
 // batches: box = blockIdx.z; per-box element strides sbox (state/K) and ibox (A4)
 struct P5Args {
     const void* in;   // A4
-    double2* uh;      // state
-    const void* k1;   // K1..K3 (bulk precision)
-    const void* k2;
-    const void* k3;
-    void* kout;       // K_s output (stage < 4)
+    double2* uh;      // state at the start of the step
+    double2* unew;    // Dormand-Prince final stage: candidate state (never uh), or null (in place)
+    StageRow row;     // K_1..K_6 (bulk precision) and the stage's tableau row (Dormand-Prince)
+    void* kout;       // K_s output (stage < nstage)
+    double* errpart;  // Dormand-Prince final stage: per-block error sums [box][slot][3], or null
     const double2* extra;  // optional term added to every stage's nonlinear term (state layout)
     const void* twN;
     const DevScalars* sc;
     const double* fslots;
     double kf2, eps_in, norm;
     long long sbox, ibox;
-    int visc_pow, scheme, stage, force;
+    int visc_pow, scheme, stage, nstage, force, nslots;
     int kmax, NK, NZP;
 };
 
-template <typename T, int N, int EPT, int W, int MODE>
+// Dormand-Prince 5(4) weights (Solver/src/solver.c:41-42) and error weights B_i - Bs_i
+constexpr double DPB1 = 35.0 / 384.0, DPB3 = 500.0 / 1113.0, DPB4 = 125.0 / 192.0, DPB5 = -2187.0 / 6784.0,
+                 DPB6 = 11.0 / 84.0;
+constexpr double DPE1 = DPB1 - 5179.0 / 57600.0, DPE3 = DPB3 - 7571.0 / 16695.0, DPE4 = DPB4 - 393.0 / 640.0,
+                 DPE5 = DPB5 - (-92097.0 / 339200.0), DPE6 = DPB6 - 187.0 / 2100.0, DPE7 = -1.0 / 40.0;
+// error tolerances (Solver/src/data_types.h:123-124)
+constexpr double DP_ATOL = 1e-7, DP_RTOL = 1e-7;
+
+template <typename T, int N, int EPT, int W, int MODE, int FAM>
 __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
     using C = cpx_t<T>;
     constexpr int TPF = N / EPT;
@@ -684,9 +727,10 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
     const int box = blockIdx.z;
     const DevScalars& sc = a.sc[box];
     double2* __restrict__ uh = a.uh + box * a.sbox;
-    const C* K1 = (const C*)a.k1 + box * a.sbox;
-    const C* K2 = (const C*)a.k2 + box * a.sbox;
-    const C* K3 = (const C*)a.k3 + box * a.sbox;
+    double2* __restrict__ unew = a.unew ? a.unew + box * a.sbox : nullptr;  // null: update in place
+    const C* K1 = (const C*)a.row.k[0] + box * a.sbox;
+    const C* K2 = (const C*)a.row.k[1] + box * a.sbox;
+    const C* K3 = (const C*)a.row.k[2] + box * a.sbox;
     C* ko = a.kout ? (C*)a.kout + box * a.sbox : nullptr;
     const double2* extra = a.extra ? a.extra + box * a.sbox : nullptr;
     const double* fslots = a.fslots + box * a.NK;
@@ -720,6 +764,7 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
     }
     const double dt = sc.dt, nu = sc.nu;
     const long long kcut2 = (long long)kmax * kmax;
+    double es[3] = {0.0, 0.0, 0.0};  // Dormand-Prince error sums of this thread
 
 #pragma unroll
     for (int m = 0; m < EPT; ++m) {
@@ -754,18 +799,30 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
         }
         const double D = visc_rate(nu, a.visc_pow, k2);
         if (a.force && keep && k2 > 0 && (double)k2 <= a.kf2) {
-            double al, be;
-            stage_in_coef(a.scheme, a.stage, D, dt, al, be);
-            const C* kp = (a.stage == 2) ? K1 : (a.stage == 3) ? K2 : K3;
+            if constexpr (FAM == 1) {
+                double2 us[3];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                double2 us = uh[c * plane + idx];
-                if (a.stage > 1) {
-                    double2 kv = to_d2<T>(kp[c * plane + idx]);
-                    us = make_double2(al * us.x + be * kv.x, al * us.y + be * kv.y);
+                for (int c = 0; c < 3; ++c) us[c] = uh[c * plane + idx];
+                add_slopes<T>(us, a.row, a.stage - 1, box * a.sbox, idx, plane, dt);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    nl[c].x += ffac * us[c].x;
+                    nl[c].y += ffac * us[c].y;
                 }
-                nl[c].x += ffac * us.x;
-                nl[c].y += ffac * us.y;
+            } else {
+                double al, be;
+                stage_in_coef(a.scheme, a.stage, D, dt, al, be);
+                const C* kp = (a.stage == 2) ? K1 : (a.stage == 3) ? K2 : K3;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    double2 us = uh[c * plane + idx];
+                    if (a.stage > 1) {
+                        double2 kv = to_d2<T>(kp[c * plane + idx]);
+                        us = make_double2(al * us.x + be * kv.x, al * us.y + be * kv.y);
+                    }
+                    nl[c].x += ffac * us.x;
+                    nl[c].y += ffac * us.y;
+                }
             }
         }
         if (extra && keep) {
@@ -776,9 +833,44 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
                 nl[c].y += e.y;
             }
         }
-        if (a.stage < 4) {
+        if (a.stage < a.nstage) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) ko[c * plane + idx] = from_d2<T>(nl[c]);
+        } else if constexpr (FAM == 1) {
+            // u (2-D)/(2+D) + 2dt/(2+D) sum_i B_i K_i (Solver/src/solver.c:336-349); Dormand-Prince:
+            // the candidate goes to unew and e = 2dt/(2+D) sum_i (B_i - Bs_i) K_i is measured
+            // against atol + max(|u|, |u5|) rtol per component (:356-384, estimate corrected)
+            const double Dd = D * dt;
+            const double g = (2.0 - Dd) / (2.0 + Dd), h = 2.0 * dt / (2.0 + Dd);
+            const C* K4 = (const C*)a.row.k[3] + box * a.sbox;
+            const C* K5 = (const C*)a.row.k[4] + box * a.sbox;
+            const bool dp = a.nstage == 7;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double2 u0 = uh[c * plane + idx];
+                const double2 q1 = to_d2<T>(K1[c * plane + idx]);
+                const double2 q3 = to_d2<T>(K3[c * plane + idx]);
+                const double2 q4 = to_d2<T>(K4[c * plane + idx]);
+                const double2 q5 = to_d2<T>(K5[c * plane + idx]);
+                double2 q6 = nl[c];
+                if (dp) q6 = to_d2<T>(((const C*)a.row.k[5] + box * a.sbox)[c * plane + idx]);
+                double2 un;
+                un.x = u0.x * g + h * (DPB1 * q1.x + DPB3 * q3.x + DPB4 * q4.x + DPB5 * q5.x + DPB6 * q6.x);
+                un.y = u0.y * g + h * (DPB1 * q1.y + DPB3 * q3.y + DPB4 * q4.y + DPB5 * q5.y + DPB6 * q6.y);
+                if (!keep) un = make_double2(0.0, 0.0);
+                if (dp) {
+                    double ex = DPE1 * q1.x + DPE3 * q3.x + DPE4 * q4.x + DPE5 * q5.x + DPE6 * q6.x + DPE7 * nl[c].x;
+                    double ey = DPE1 * q1.y + DPE3 * q3.y + DPE4 * q4.y + DPE5 * q5.y + DPE6 * q6.y + DPE7 * nl[c].y;
+                    ex *= h;
+                    ey *= h;
+                    const double a0 = sqrt(u0.x * u0.x + u0.y * u0.y), a1 = sqrt(un.x * un.x + un.y * un.y);
+                    const double den = DP_ATOL + (a0 > a1 ? a0 : a1) * DP_RTOL;
+                    if (keep) es[c] += (ex * ex + ey * ey) / (den * den);
+                    unew[c * plane + idx] = un;
+                } else {
+                    uh[c * plane + idx] = un;
+                }
+            }
         } else {
             constexpr double B1 = 1.0 / 6.0, B2 = 1.0 / 3.0, B3 = 1.0 / 3.0, B4 = 1.0 / 6.0;
             double g, h = 0.0, e2 = 0.0;
@@ -811,6 +903,17 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
             }
         }
     }
+    if (FAM == 1 && MODE == EP_STAGE && a.errpart) {
+        // per-block error sums, reduced in a fixed order by dp_finalize_kernel
+        __syncthreads();  // the transforms are done with lds
+        double* red = (double*)lds;
+        double* dst = a.errpart + ((long long)box * a.nslots + (long long)iy * (NZP / W) + tile) * 3;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double v = block_sum(es[c], red);
+            if (tid == 0) dst[c] = v;
+        }
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -830,6 +933,38 @@ __global__ void step_end_kernel(DevScalars* sc, int nbox) {
         sc[b].t += sc[b].dt;
         sc[b].steps += 1;
         sc[b].max_last = __longlong_as_double((long long)sc[b].maxbits);
+    }
+}
+
+// Dormand-Prince step control (Solver/src/solver.c:134-153, constants Solver/src/data_types.h:125-127):
+// err = max_c sqrt(sum_c / (Nx Ny Nzf)); accepted when err <= 1 (then t += dt); in every case
+// dt <- dt min(1.5, max(0.01, 0.8 err^(-1/5))), with the reference's max/min, which pass a NaN on.
+// One block per box; the per-block sums are added in a fixed order.
+__device__ __forceinline__ double dp_max(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ double dp_min(double a, double b) { return a < b ? a : b; }
+__global__ void __launch_bounds__(256) dp_finalize_kernel(DevScalars* sc, const double* errpart, int nslots, double count) {
+    __shared__ double red[256];
+    const int box = blockIdx.x;
+    const double* part = errpart + (long long)box * nslots * 3;
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int j = threadIdx.x; j < nslots; j += 256) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) s[c] += part[3 * j + c];
+    }
+    double err = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) err = nanmax(err, sqrt(block_sum(s[c], red) / count));
+    if (threadIdx.x == 0) {
+        DevScalars& d = sc[box];
+        const double h = d.dt;
+        d.dperr = err;
+        d.dpacc = err <= 1.0 ? 1 : 0;
+        if (err <= 1.0) {
+            d.t += h;
+            d.steps += 1;
+            d.max_last = __longlong_as_double((long long)d.maxbits);
+        }
+        d.dt = h * dp_min(1.5, dp_max(0.01, 0.8 * pow(1.0 / err, 0.2)));
     }
 }
 

@@ -16,6 +16,10 @@ evaluation at 512^3, scaled to steps/s as 1/(4 x that time).
 32^3, 32 at 48^3, 24 at 64^3, 8 at 96^3), 100 RK4 steps, autotuned FFT schedules, one HIP graph
 replay per ensemble step, against the same steps launched box by box.
 
+"integrators" times the Dormand-Prince schemes on the same forced 512^3 workload: RK5-CN (six
+stages, fixed dt = the RK4 run's mean CFL step) in FP64 and FP32-bulk, and the adaptive DP5 loop
+(seven stages per try, FP64) from that dt.
+
 "closure_les" times the learned-closure LES on the 64^3 grid (the forced DNS state filtered with
 cutoff 21, the BF16 MFMA network evaluated every step) and reports the ratio of GPU seconds per
 unit of simulated time, DNS over LES, plus the network's forward throughput against the MI355X
@@ -98,6 +102,45 @@ def run_mode(nsg, N, precision, steps, warmup, prof_iters):
     out = {"steps_per_s": steps / el, "ms_per_step": 1e3 * el / steps, "mean_dt": mean_dt, "roofline": roof, "passes": passes,
            "step_bytes_GB": step_gb, "step_GBps": step_gb / (el / steps)}
     return s, out
+
+
+def run_integrators(nsg, N, steps, warmup, dt_cfl):
+    """the Dormand-Prince integrators on the same forced workload: RK5-CN (fixed dt = the RK4 run's
+    mean CFL dt) in FP64 and FP32-bulk, and adaptive DP5 in FP64 from that dt (accepted steps and
+    simulated time per second, rejections included)"""
+    out = {}
+    for prec in ("fp64", "fp32bulk"):
+        s = nsg.Solver(N, nu=1e-3, dt=dt_cfl, scheme="rk5cn", precision=prec, force_eps=0.1, force_kf=2.5)
+        s.init("k4", k0=2.0, energy=0.5, seed=2)
+        s.step(warmup)
+        t0 = time.perf_counter()
+        s.step(steps)
+        el = time.perf_counter() - t0
+        ms, by = s.profile(1)
+        gb = s.step_bytes() / 1e9
+        out[f"rk5cn_{prec}"] = {"steps_per_s": steps / el, "ms_per_step": 1e3 * el / steps, "step_bytes_GB": gb,
+                                "step_GBps": gb / (el / steps),
+                                "pass_GBps": {n: by[i] / (ms[i] * 1e-3) / 1e9 for i, n in enumerate(PASS_NAMES)}}
+        log(f"[bench] rk5cn {prec}: {steps / el:.2f} steps/s, {gb / (el / steps):.0f} GB/s over the step")
+        s.close()
+    s = nsg.Solver(N, nu=1e-3, dt=dt_cfl, scheme="dp5", force_eps=0.1, force_kf=2.5)
+    s.init("k4", k0=2.0, energy=0.5, seed=2)
+    s.step(warmup)
+    a0, r0, _ = s.dp_stats()
+    tb = s.time()[0]
+    n = max(2, steps // 2)
+    t0 = time.perf_counter()
+    s.step(n)
+    el = time.perf_counter() - t0
+    a1, r1, err = s.dp_stats()
+    t1, dt1, _ = s.time()
+    out["dp5_fp64"] = {"accepted_steps_per_s": n / el, "tries_per_step": (n + r1 - r0) / n,
+                       "sim_time_per_s": (t1 - tb) / el, "mean_dt": (t1 - tb) / n, "dt_next": dt1, "last_err": err,
+                       "rk4_cfl_dt": dt_cfl}
+    log(f"[bench] dp5 fp64: {n / el:.2f} accepted steps/s, {(n + r1 - r0) / n:.2f} tries/step, mean dt "
+        f"{(t1 - tb) / n:.3e} (RK4 CFL dt {dt_cfl:.3e})")
+    s.close()
+    return out
 
 
 ENSEMBLE_SIZES = [32] * 32 + [48] * 32 + [64] * 24 + [96] * 8
@@ -215,6 +258,8 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=512)
     ap.add_argument("--ensemble-steps", type=int, default=100, help="0 skips the 96-box ensemble")
     ap.add_argument("--no-closure", dest="closure", action="store_false", help="skip the closure-LES timing")
+    ap.add_argument("--no-integrators", dest="integrators", action="store_false",
+                    help="skip the RK5-CN / Dormand-Prince timing")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -261,6 +306,7 @@ def main():
     s64.close()
     clos = run_closure(nsg, s32, r32) if args.closure else None
     s32.close()
+    integ = run_integrators(nsg, N, max(4, args.steps // 2), 2, r64["mean_dt"]) if args.integrators else None
 
     ens = run_ensemble(nsg, args.ensemble_steps, 3) if args.ensemble_steps > 0 else None
 
@@ -299,6 +345,7 @@ def main():
         "passes": r32["passes"],
         "step_bytes_GB": r32["step_bytes_GB"],
         "step_GBps": r32["step_GBps"],
+        "integrators": integ,
         "ensemble_96_boxes": ens,
         "closure_les": clos,
         "cpu_baseline": cpu,

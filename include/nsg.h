@@ -25,9 +25,15 @@ extern "C" {
 #define NSG_ERR_DEVICE (-3)    /* HIP runtime error */
 #define NSG_ERR_NONFINITE (-4) /* NaN/Inf in the state or its diagnostics */
 #define NSG_ERR_BLOWUP (-5)    /* max|u| exceeded nsg_params.max_speed */
+#define NSG_ERR_STEP (-6)      /* an adaptive step was rejected 50 times (Solver/src/data_types.h:128) */
 
 #define NSG_SCHEME_RK4CN 0 /* reference RK4 with Crank-Nicolson viscous factor, Solver/src/solver.c:505-608 */
 #define NSG_SCHEME_IFRK4 1 /* integrating-factor RK4 */
+#define NSG_SCHEME_RK5CN 2 /* reference RK5 (-D__RK5): Dormand-Prince 5th-order weights, Crank-Nicolson
+                              factor, fixed dt, Solver/src/solver.c:217-349 (A32 = 9/40, see nsg_step) */
+#define NSG_SCHEME_DP5 3   /* reference adaptive Dormand-Prince (-D__DPRK5), Solver/src/solver.c:134-153,
+                              217-421; one box per context, FP64 only, cfl must be 0, dt is the
+                              first try's step */
 
 #define NSG_FP64 0     /* every FFT and product in FP64 */
 #define NSG_FP32BULK 1 /* FFTs, products and stage increments in FP32; state, RK update,
@@ -82,8 +88,20 @@ int nsg_init_state(nsg_ctx* ctx, int kind, double k0, double energy, uint64_t se
 /* NonlinearRHSBatch (Solver/src/solver.c:620-731), plus forcing when enabled:
  * out = dealias(P(r2c(c2r(u_hat) x c2r(i k x u_hat)))/N^6) + f(u_hat) */
 int nsg_rhs(nsg_ctx* ctx, const double* u_hat, double* out);
-/* nsteps x RK4Step (Solver/src/solver.c:131 -> :505) on the device-resident state */
+/* nsteps x RK4Step (Solver/src/solver.c:131 -> :505) on the device-resident state; RK5CN: RK5DPStep
+ * (:133 -> :217).  DP5: nsteps accepted adaptive steps (the loop of :134-153); each try runs the
+ * seven stages, the embedded error err = max_c sqrt(sum |e_c/(1e-7 + 1e-7 max(|u_c|,|u5_c|))|^2 /
+ * (N^2 (N/2+1))) with e = 2dt/(2+D) sum (B_i - Bs_i) K_i, is accepted when err <= 1 (time advances
+ * by that try's dt), and sets dt <- dt min(1.5, max(0.01, 0.8 err^-1/5)) either way; 50 rejections
+ * in a row fail with NSG_ERR_STEP.  Deliberate differences from the reference's code: A32 = 9/40
+ * (the reference's 0.5 makes the scheme first order), the error taken between the 5th- and
+ * 4th-order solutions from the same start state with |e|^2 and zeroed sums, the accept test
+ * err <= 1 (the reference retries exactly those tries), t = sum of accepted dt, and the first
+ * step checked too. */
 int nsg_step(nsg_ctx* ctx, long nsteps);
+/* Dormand-Prince bookkeeping (RK_data->DP_err, DP_fails, Solver/src/data_types.h:233-234): steps
+ * accepted, tries rejected, error norm of the last try; NSG_ERR_INVALID for other schemes */
+int nsg_dp_stats(nsg_ctx* ctx, long long* accepted, long long* rejected, double* last_err);
 /* ComputeSystemMeasurables (Solver/src/solver.c:1142-1276); E_k/Ens_k may be NULL, else n_spect
  * doubles. quirk = 1 reproduces the reference's operator-precedence (D3) and conjugate-free
  * helicity (D14) arithmetic. */

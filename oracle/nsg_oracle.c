@@ -401,6 +401,9 @@ static void rhs_forced(int N, const double* uh, double* nl, double eps, double k
     oracle_forcing(N, uh, eps, kf, nl);
 }
 
+double oracle_step_rk5(int N, const double* uhd, double* outd, double nu, int visc_pow, double dt, int nstage,
+                       int quirk, double force_eps, double force_kf);
+
 /* nu |k|^{2p}: reference __HYPER uses pow(k^2, 2.0) (Solver/src/data_types.h:68-71) */
 static inline double visc_rate(double nu, int p, long k2) {
     double kk = (double)k2;
@@ -416,6 +419,11 @@ static inline double visc_rate(double nu, int p, long k2) {
 /* ------------------------------------------------------------------------------------------ */
 void oracle_step(int N, double* uhd, double nu, int visc_pow, double dt, int scheme,
                  double force_eps, double force_kf) {
+    if (scheme == ORACLE_RK5CN || scheme == ORACLE_DP5) {
+        /* one fixed-dt step; the fifth-order update does not use the seventh stage */
+        oracle_step_rk5(N, uhd, uhd, nu, visc_pow, dt, 6, 0, force_eps, force_kf);
+        return;
+    }
     const int Nzf = N / 2 + 1;
     const long C = (long)N * N * Nzf;
     const long n3 = 3 * C;
@@ -490,6 +498,159 @@ void oracle_step(int N, double* uhd, double nu, int visc_pow, double dt, int sch
         }
     }
     free(K1); free(K2); free(K3); free(K4); free(tmp); free(E); free(E2);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* RK5DPStep, Solver/src/solver.c:217-421 (tableau :34-43), __NAVIER branch: stage s input     */
+/* u + dt sum_j A_sj K_j (no viscous factor inside the stages), update                         */
+/* u (2-D)/(2+D) + 2dt/(2+D) sum_i B_i K_i with D = nu|k|^2p dt.  nstage 6 is the reference's   */
+/* fixed-step __RK5 build; nstage 7 adds the seventh (FSAL-row) stage of __DPRK5 and returns    */
+/* the embedded error norm.                                                                     */
+/*   D18: the reference has A32 = 0.5; the published Dormand-Prince 5(4) tableau (and the      */
+/*        consistency condition sum_j A_3j = C3 = 3/10) needs 9/40.  With 0.5 the method is    */
+/*        first order; quirk = 1 reproduces it.                                                 */
+/*   D19: error e = u5 - u4 = 2dt/(2+D) sum_i (B_i - Bs_i) K_i, both solutions from the same   */
+/*        start state (the reference adds dt sum Bs K to the already-updated state, so its     */
+/*        "error" is the step's increment, :356-359), |e|^2 (the reference's cpow(z, 2.0) keeps */
+/*        Re z^2, :367-369), sums initialised to zero (uninitialised in the reference) and the  */
+/*        first step checked too (the reference skips iters == 1, :353).                        */
+/*   Norm as the reference: err = max_c sqrt(sum |e_c / (atol + max(|u_c|, |u5_c|) rtol)|^2 /  */
+/*   (Nx Ny Nzf)) over every stored half-spectrum entry (:382-408), atol = rtol = 1e-7         */
+/*   (Solver/src/data_types.h:123-124).                                                        */
+/* ------------------------------------------------------------------------------------------ */
+#define DP_ATOL 1e-7
+#define DP_RTOL 1e-7
+static void dp_tableau(int quirk, double A[7][6], double B[7], double Bs[7]) {
+    memset(A, 0, sizeof(double) * 42);
+    A[1][0] = 1.0 / 5.0;
+    A[2][0] = 3.0 / 40.0;       A[2][1] = quirk ? 0.5 : 9.0 / 40.0;
+    A[3][0] = 44.0 / 45.0;      A[3][1] = -56.0 / 15.0;      A[3][2] = 32.0 / 9.0;
+    A[4][0] = 19372.0 / 6561.0; A[4][1] = -25360.0 / 2187.0; A[4][2] = 64448.0 / 6561.0; A[4][3] = -212.0 / 729.0;
+    A[5][0] = 9017.0 / 3168.0;  A[5][1] = -355.0 / 33.0;     A[5][2] = 46732.0 / 5247.0; A[5][3] = 49.0 / 176.0;
+    A[5][4] = -5103.0 / 18656.0;
+    A[6][0] = 35.0 / 384.0;     A[6][2] = 500.0 / 1113.0;    A[6][3] = 125.0 / 192.0;    A[6][4] = -2187.0 / 6784.0;
+    A[6][5] = 11.0 / 84.0;
+    const double b[7] = {35.0 / 384.0, 0.0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0, 0.0};
+    const double bs[7] = {5179.0 / 57600.0, 0.0, 7571.0 / 16695.0, 393.0 / 640.0, -92097.0 / 339200.0,
+                          187.0 / 2100.0, 1.0 / 40.0};
+    for (int i = 0; i < 7; ++i) { B[i] = b[i]; Bs[i] = bs[i]; }
+}
+
+static inline double cabs2(cplx z) { return z.re * z.re + z.im * z.im; }
+
+double oracle_step_rk5(int N, const double* uhd, double* outd, double nu, int visc_pow, double dt, int nstage,
+                       int quirk, double force_eps, double force_kf) {
+    const int Nzf = N / 2 + 1;
+    const long C = (long)N * N * Nzf;
+    const long n3 = 3 * C;
+    const cplx* uh = (const cplx*)uhd;
+    cplx* out = (cplx*)outd;
+    double A[7][6], B[7], Bs[7];
+    dp_tableau(quirk, A, B, Bs);
+    cplx* K[7];
+    for (int s = 0; s < nstage; ++s) K[s] = (cplx*)malloc(sizeof(cplx) * (size_t)n3);
+    cplx* tmp = (cplx*)malloc(sizeof(cplx) * (size_t)n3);
+    for (int s = 0; s < nstage; ++s) {
+        if (s == 0) {
+            rhs_forced(N, uhd, (double*)K[0], force_eps, force_kf);
+            continue;
+        }
+#pragma omp parallel for schedule(static)
+        for (long m = 0; m < n3; ++m) {
+            cplx v = uh[m];
+            for (int j = 0; j < s; ++j) {
+                if (A[s][j] == 0.0) continue;
+                v.re += dt * A[s][j] * K[j][m].re;
+                v.im += dt * A[s][j] * K[j][m].im;
+            }
+            tmp[m] = v;
+        }
+        rhs_forced(N, (const double*)tmp, (double*)K[s], force_eps, force_kf);
+    }
+    double esum[3] = {0.0, 0.0, 0.0};
+    double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : e0, e1, e2)
+    for (long m = 0; m < n3; ++m) {
+        long n = m / 3;
+        long i = n / ((long)N * Nzf), j = (n / Nzf) % N, k = n % Nzf;
+        long kx = kxy_of((int)i, N), ky = kxy_of((int)j, N);
+        const double D = visc_rate(nu, visc_pow, kx * kx + ky * ky + k * k) * dt;
+        const double g = (2.0 - D) / (2.0 + D), h = 2.0 * dt / (2.0 + D);
+        const cplx u0 = uh[m];
+        cplx acc = {0.0, 0.0};
+        for (int q = 0; q < 6; ++q) {
+            if (B[q] == 0.0) continue;
+            acc.re += B[q] * K[q][m].re;
+            acc.im += B[q] * K[q][m].im;
+        }
+        cplx un = {u0.re * g + h * acc.re, u0.im * g + h * acc.im};
+        if (nstage == 7) {
+            cplx e = {0.0, 0.0};
+            for (int q = 0; q < 7; ++q) {
+                const double w = B[q] - Bs[q];
+                if (w == 0.0) continue;
+                e.re += w * K[q][m].re;
+                e.im += w * K[q][m].im;
+            }
+            e.re *= h;
+            e.im *= h;
+            double a0 = sqrt(cabs2(u0)), a1 = sqrt(cabs2(un));
+            const double den = DP_ATOL + (a0 > a1 ? a0 : a1) * DP_RTOL;
+            const double r = cabs2(e) / (den * den);
+            if (m % 3 == 0) e0 += r;
+            else if (m % 3 == 1) e1 += r;
+            else e2 += r;
+        }
+        out[m] = un;
+    }
+    esum[0] = e0; esum[1] = e1; esum[2] = e2;
+    for (int s = 0; s < nstage; ++s) free(K[s]);
+    free(tmp);
+    if (nstage != 7) return 0.0;
+    double err = 0.0;
+    for (int c = 0; c < 3; ++c) {
+        const double ec = sqrt(esum[c] / (double)C);
+        if (ec > err || ec != ec) err = ec;
+    }
+    return err;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Dormand-Prince step-size control, Solver/src/solver.c:134-153 (constants                     */
+/* Solver/src/data_types.h:123-128): dt_new = dt min(1.5, max(0.01, 0.8 err^(-1/5))), at most   */
+/* 50 tries per step.  D20: a try is accepted when err <= 1 (the reference retries exactly      */
+/* those, :143) and time advances by the accepted dt (the reference sets t = iters * dt, :194). */
+/* Returns 0, or -1 when a step was rejected 50 times or the error is not finite.               */
+/* ------------------------------------------------------------------------------------------ */
+static inline double dp_max(double a, double b) { return a > b ? a : b; }
+static inline double dp_min(double a, double b) { return a < b ? a : b; }
+
+double oracle_dp5_factor(double err) { return dp_min(1.5, dp_max(0.01, 0.8 * pow(1.0 / err, 0.2))); }
+
+int oracle_dp5_solve(int N, double* uhd, double nu, int visc_pow, double* dt, double* t, int nsteps,
+                     double force_eps, double force_kf, long* rejected, double* last_err) {
+    const long n3 = 3L * N * N * (N / 2 + 1);
+    double* cand = (double*)malloc(sizeof(double) * 2 * (size_t)n3);
+    int rc = 0;
+    for (int s = 0; s < nsteps && rc == 0; ++s) {
+        int tries = 0;
+        for (;;) {
+            const double err = oracle_step_rk5(N, uhd, cand, nu, visc_pow, *dt, 7, 0, force_eps, force_kf);
+            if (last_err) *last_err = err;
+            const double h = *dt;
+            *dt = h * oracle_dp5_factor(err);
+            if (!(err == err) || !isfinite(*dt)) { rc = -1; break; }
+            if (err <= 1.0) {
+                memcpy(uhd, cand, sizeof(double) * 2 * (size_t)n3);
+                *t += h;
+                break;
+            }
+            if (rejected) ++*rejected;
+            if (++tries >= 50) { rc = -1; break; }
+        }
+    }
+    free(cand);
+    return rc;
 }
 
 /* ------------------------------------------------------------------------------------------ */

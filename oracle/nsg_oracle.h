@@ -25,6 +25,8 @@
  * Parity pins: every function is checked by tests/test_oracle_*.py against numpy golden vectors
  * (tests/golden/), brute-force DFTs, closed forms (Taylor-Green, Beltrami/Shapiro) or invariants.
  * oracle_ic_random_ref is "parity unpinned" (the reference's rand() draw order is unspecified C).
+ * D18-D20 (Dormand-Prince tableau, error estimate and step control) are listed at oracle_step_rk5
+ * and oracle_dp5_solve in nsg_oracle.c.
  */
 #ifndef NSG_ORACLE_H
 #define NSG_ORACLE_H
@@ -36,6 +38,8 @@ extern "C" {
 /* time-stepping schemes */
 #define ORACLE_RK4CN 0 /* reference RK4 + Crank-Nicolson viscous factor, Solver/src/solver.c:505-608 */
 #define ORACLE_IFRK4 1 /* integrating-factor RK4 (exp(-nu|k|^2p dt)), no reference counterpart */
+#define ORACLE_RK5CN 2 /* Dormand-Prince 5th-order weights + Crank-Nicolson factor, Solver/src/solver.c:217-421 (__RK5) */
+#define ORACLE_DP5 3   /* same update; the adaptive controller is oracle_dp5_solve (__DPRK5) */
 
 int oracle_n_spect(int N);
 void oracle_wavenumbers(int N, int* kx, int* ky, int* kz);
@@ -53,6 +57,15 @@ void oracle_rhs(int N, const double* uh, double* nl);
 void oracle_forcing(int N, const double* uh, double eps_in, double kf, double* nl);
 void oracle_step(int N, double* uh, double nu, int visc_pow, double dt, int scheme,
                  double force_eps, double force_kf);
+/* one Dormand-Prince RK step from uh into out (may alias uh); nstage 6 (__RK5) or 7 (__DPRK5,
+   returns the embedded error norm, 0 otherwise); quirk 1 uses the reference's A32 = 0.5 (D18) */
+double oracle_step_rk5(int N, const double* uh, double* out, double nu, int visc_pow, double dt, int nstage,
+                       int quirk, double force_eps, double force_kf);
+/* step-size factor min(1.5, max(0.01, 0.8 err^(-1/5))) */
+double oracle_dp5_factor(double err);
+/* nsteps accepted adaptive steps; dt and t updated in place; 0 or -1 (50 rejections / non-finite) */
+int oracle_dp5_solve(int N, double* uh, double nu, int visc_pow, double* dt, double* t, int nsteps,
+                     double force_eps, double force_kf, long* rejected, double* last_err);
 void oracle_measures(int N, const double* uh, double nu, int visc_pow, int quirk,
                      double* totals5, double* Ek, double* Ensk);
 void oracle_realspace(int N, const double* uh, int which, double* out);

@@ -48,6 +48,12 @@ def load():
     lib.oracle_ic_random_ref.argtypes = [i, vp]
     lib.oracle_ic_k4.argtypes = [i, d, d, ctypes.c_uint64, vp]
     lib.oracle_philox4x32_10.argtypes = [vp, vp, vp]
+    lib.oracle_step_rk5.argtypes = [i, vp, vp, d, i, d, i, i, d, d]
+    lib.oracle_step_rk5.restype = d
+    lib.oracle_dp5_factor.argtypes = [d]
+    lib.oracle_dp5_factor.restype = d
+    lib.oracle_dp5_solve.argtypes = [i, vp, d, i, vp, vp, i, d, d, vp, vp]
+    lib.oracle_dp5_solve.restype = i
     _lib = lib
     return lib
 
@@ -106,11 +112,34 @@ def rhs(uh, force_eps=0.0, force_kf=2.5):
 
 def step(uh, nu, dt, visc_pow=1, scheme="rk4cn", force_eps=0.0, force_kf=2.5, nsteps=1):
     a = np.ascontiguousarray(uh, dtype=np.complex128).copy()
-    sc = {"rk4cn": 0, "ifrk4": 1}[scheme]
+    sc = {"rk4cn": 0, "ifrk4": 1, "rk5cn": 2, "dp5": 3}[scheme]
     for _ in range(nsteps):
         load().oracle_step(a.shape[0], _p(a), float(nu), int(visc_pow), float(dt), sc, float(force_eps),
                            float(force_kf))
     return a
+
+
+def step_rk5(uh, nu, dt, nstage=6, visc_pow=1, quirk=False, force_eps=0.0, force_kf=2.5):
+    """one Dormand-Prince RK step; returns (u5, embedded error norm or 0 for nstage 6)"""
+    a = np.ascontiguousarray(uh, dtype=np.complex128)
+    out = np.empty_like(a)
+    err = load().oracle_step_rk5(a.shape[0], _p(a), _p(out), float(nu), int(visc_pow), float(dt), int(nstage),
+                                 int(quirk), float(force_eps), float(force_kf))
+    return out, err
+
+
+def dp5_factor(err):
+    return load().oracle_dp5_factor(float(err))
+
+
+def dp5_solve(uh, nu, dt, nsteps, t=0.0, visc_pow=1, force_eps=0.0, force_kf=2.5):
+    """nsteps accepted adaptive Dormand-Prince steps: (u, t, dt_next, rejected, last_err, rc)"""
+    a = np.ascontiguousarray(uh, dtype=np.complex128).copy()
+    dtv, tv = np.array([dt], dtype=np.float64), np.array([t], dtype=np.float64)
+    rej, last = np.zeros(1, dtype=np.int64), np.zeros(1)
+    rc = load().oracle_dp5_solve(a.shape[0], _p(a), float(nu), int(visc_pow), _p(dtv), _p(tv), int(nsteps),
+                                 float(force_eps), float(force_kf), _p(rej), _p(last))
+    return a, float(tv[0]), float(dtv[0]), int(rej[0]), float(last[0]), rc
 
 
 def measures(uh, nu, visc_pow=1, quirk=False):

@@ -40,7 +40,9 @@ def test_default_params_mirror_reference_cli_defaults(nsg):
 
 @pytest.mark.parametrize("kw,msg", [({"N": 100}, "unsupported N"), ({"N": 64, "dt": 0.0}, "dt"),
                                     ({"N": 64, "visc_pow": 7}, "visc_pow"),
-                                    ({"N": 64, "force_eps": 0.1, "force_kf": 9.0}, "force_kf")])
+                                    ({"N": 64, "force_eps": 0.1, "force_kf": 9.0}, "force_kf"),
+                                    ({"N": 64, "scheme": "dp5", "cfl": 0.5}, "cfl must be 0"),
+                                    ({"N": 64, "scheme": "dp5", "precision": "fp32bulk"}, "FP32 round-off")])
 def test_invalid_parameters_are_rejected_before_device_use(nsg, kw, msg):
     N = kw.pop("N")
     with pytest.raises(nsg.NSGError) as e:
@@ -57,3 +59,10 @@ def test_non_cubic_grid_rejected(nsg):
     assert lib.nsg_create(ctypes.byref(p), ctypes.byref(h)) == -1
     assert b"cubic" in lib.nsg_last_error(h)
     lib.nsg_destroy(h)
+
+
+def test_dormand_prince_is_single_box(nsg):
+    """the adaptive scheme accepts or rejects each box's try on the host: no ensembles of it"""
+    with pytest.raises(nsg.NSGError) as e:
+        nsg.Ensemble([dict(N=16, scheme="dp5"), dict(N=16, scheme="dp5")])
+    assert e.value.code == -1 and "one box per context" in str(e.value)
