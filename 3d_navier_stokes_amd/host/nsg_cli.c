@@ -27,7 +27,7 @@ void nsg_print_usage(const char* prog) {
             "  -z FILE    restart from the last /Iter_* group of a Main_HDF_Data.h5 file\n"
             "  -d ALPHA   Ekman drag (accepted and unused, as in the reference)\n"
             "  --visc-pow P   1: nu|k|^2, 2: nu|k|^4 (reference -D__HYPER)\n"
-            "  --scheme S     rk4cn (reference) | ifrk4\n"
+            "  --scheme S     rk4cn (reference) | ifrk4 | rk5cn (-D__RK5) | dp5 (-D__DPRK5, adaptive)\n"
             "  --precision P  fp64 | fp32bulk\n"
             "  --adaptive     CFL-adaptive step with the -c constant\n"
             "  --force-eps E  constant-power forcing rate (with -f CONST_POWER; default 0.1)\n"
@@ -142,6 +142,8 @@ int nsg_parse_args(int argc, char** argv, nsg_run* r) {
             case 1001:
                 if (!strcmp(optarg, "rk4cn")) r->scheme = NSG_SCHEME_RK4CN;
                 else if (!strcmp(optarg, "ifrk4")) r->scheme = NSG_SCHEME_IFRK4;
+                else if (!strcmp(optarg, "rk5cn")) r->scheme = NSG_SCHEME_RK5CN;
+                else if (!strcmp(optarg, "dp5")) r->scheme = NSG_SCHEME_DP5;
                 else return fail("unknown --scheme", 0);
                 break;
             case 1002:

@@ -181,7 +181,9 @@ int main(int argc, char** argv) {
     /* integration counters (InitializeIntegrationVariables, Solver/src/solver.c:1314-1363) */
     const long num_t_steps = (long)((r.T - r.t0) / r.dt);
     long num_print_steps = (num_t_steps >= r.save_every) ? num_t_steps / r.save_every + 1 : num_t_steps + 1;
-    if (r.adaptive) num_print_steps *= 2; /* room for extra saves, as __ADAPTIVE_STEP does (:1375-1379) */
+    /* the Dormand-Prince scheme chooses dt itself: step by step, like the CFL-adaptive loop */
+    const int var_dt = r.adaptive || r.scheme == NSG_SCHEME_DP5;
+    if (var_dt) num_print_steps *= 2; /* room for extra saves, as __ADAPTIVE_STEP does (:1375-1379) */
     if (!r.quiet) printf("Total Iters: %ld\t Saving Iters: %ld\n", num_t_steps, num_print_steps);
     series s = {0};
     s.cap = num_print_steps;
@@ -190,7 +192,9 @@ int main(int argc, char** argv) {
     s.Hel = calloc((size_t)s.cap, sizeof(double)); s.Diss = calloc((size_t)s.cap, sizeof(double));
 
     nsg_h5 h;
-    if (nsg_h5_open(&h, &r, "RK4") != 0) { nsg_destroy(ctx); return 1; }
+    /* solver tag of the output names (Solver/src/hdf5_funcs.c:372-380) */
+    const char* tag = r.scheme == NSG_SCHEME_RK5CN ? "RK5" : r.scheme == NSG_SCHEME_DP5 ? "DP5" : "RK4";
+    if (nsg_h5_open(&h, &r, tag) != 0) { nsg_destroy(ctx); return 1; }
     if (!r.quiet) printf("\nMain Output File: %s\n\n", h.main_name);
 
     int rc = record(ctx, &r, &h, &s, save_idx++, r.t0, dt, N, uh, wh, ur, wr, ek, ens, nspect, 0, num_t_steps, r.T);
@@ -199,7 +203,7 @@ int main(int argc, char** argv) {
     long iters = 1, pending = 0;
     while (rc == NSG_OK && t <= r.T) {
         const int save = (iters % r.save_every) == 0;
-        if (r.adaptive) {
+        if (var_dt) {
             rc = nsg_step(ctx, 1);
             if (rc == NSG_OK) nsg_get_time(ctx, &t, &dt, NULL);
         } else {
@@ -216,7 +220,7 @@ int main(int argc, char** argv) {
         }
         if (save) rc = record(ctx, &r, &h, &s, save_idx++, t, dt, N, uh, wh, ur, wr, ek, ens, nspect, iters, num_t_steps, r.T);
         iters++;
-        if (!r.adaptive) t = r.t0 + (double)iters * r.dt;
+        if (!var_dt) t = r.t0 + (double)iters * r.dt;
         else {
             double tn, dtn;
             nsg_get_time(ctx, &tn, &dtn, NULL);

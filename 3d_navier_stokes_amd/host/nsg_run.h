@@ -25,7 +25,7 @@ typedef struct nsg_run {
     char input_file[512];/* -z restart file (Main_HDF_Data.h5 written by this driver) */
     /* additions (long options) */
     int visc_pow;        /* --visc-pow 1|2 */
-    int scheme;          /* --scheme rk4cn|ifrk4 */
+    int scheme;          /* --scheme rk4cn|ifrk4|rk5cn|dp5 */
     int precision;       /* --precision fp64|fp32bulk */
     int adaptive;        /* --adaptive : CFL step with the -c constant */
     double force_eps;    /* --force-eps */

@@ -68,3 +68,20 @@ def test_realspace_datasets(oracle, tmp_path):
         assert rel(f.read("/Iter_00001/u"), oracle.realspace(uh, 0)) < 1e-12
         assert rel(f.read("/Iter_00001/w"), oracle.realspace(uh, 1)) < 1e-12
         assert rel(f.read("/Iter_00001/w_hat", "complex"), oracle.curl(uh)) < 1e-14
+
+
+def test_dormand_prince_run(oracle, tmp_path):
+    """--scheme dp5: output named like the reference's -D__DPRK5 build, every save at the accepted
+    time of the oracle's adaptive run, states equal to it"""
+    run(["-n", "32", "-n", "32", "-n", "32", "-s", "0", "-e", "0.05", "-h", "1e-3", "-v", "0.01", "-i",
+         "TAYLOR_GREEN", "-p", "1", "-t", "DP", "--no-realspace", "--scheme", "dp5"], tmp_path)
+    assert glob.glob(os.path.join(str(tmp_path), "SIM_DATA_NAVIER_DP5_FULL_*TAG[[]DP[]]"))
+    with h5read.File(main_file(tmp_path, "DP")) as f:
+        t = f.read("/Time")
+        u, tt, dt = oracle.ic_taylor_green(32), 0.0, 1e-3
+        for i in range(1, 5):
+            u, tt, dt, _, _, rc = oracle.dp5_solve(u, 0.01, dt, 1, t=tt)
+            assert rc == 0
+            assert abs(t[i] - tt) < 1e-9 * tt
+            assert abs(f.attr(f"/Iter_{i:05d}", "TimeValue") - tt) < 1e-9 * tt
+            assert rel(f.read(f"/Iter_{i:05d}/u_hat", "complex"), u) < 1e-11
