@@ -134,6 +134,20 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.nsg_attach_closure.argtypes = [vp, vp]
     lib.nsg_resample.argtypes = [vp, vp]
     lib.nsg_closure_term.argtypes = [vp, vp]
+    # increment statistics
+    lib.nsg_incr_create.argtypes = [ctypes.c_int, ctypes.c_int, P(vp)]
+    lib.nsg_incr_destroy.argtypes = [vp]
+    lib.nsg_incr_last_error.argtypes = [vp]
+    lib.nsg_incr_last_error.restype = ctypes.c_char_p
+    lib.nsg_incr_increments.argtypes = [vp, vp]
+    for name in ("nsg_incr_precompute", "nsg_incr_histogram"):
+        getattr(lib, name).argtypes = [vp, vp]
+    for name in ("nsg_incr_precompute_state", "nsg_incr_histogram_state"):
+        getattr(lib, name).argtypes = [vp, vp, ctypes.c_double]
+    lib.nsg_incr_set_ranges.argtypes = [vp]
+    lib.nsg_incr_results.argtypes = [vp, vp, vp, vp]
+    lib.nsg_incr_reset.argtypes = [vp]
+    lib.nsg_incr_time_pass.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
     _lib = lib
     return lib
 
@@ -565,4 +579,90 @@ class Network:
     def time_forward(self, batch: int = 1, iters: int = 10) -> float:
         ms = ctypes.c_double()
         self._check(self.lib.nsg_cnn_time_forward(self._h, int(batch), int(iters), ctypes.byref(ms)))
+        return ms.value
+
+
+class IncrementStats:
+    """Longitudinal velocity-increment statistics of real-space snapshots on one GPU (the
+    reference's post-processing, PostProcessing/src/stats.c): `precompute` every snapshot,
+    `set_ranges`, `histogram` every snapshot, then `results()`.  A snapshot is a real AoS field
+    (N, N, N, 3) or a Solver (its real-space velocity times `scale`)."""
+
+    NUM, BINS = 9, 1000
+
+    def __init__(self, N: int, device: int = 0):
+        self.lib = load_library()
+        h = ctypes.c_void_p()
+        rc = self.lib.nsg_incr_create(int(N), int(device), ctypes.byref(h))
+        self._h = h
+        if rc != 0:
+            msg = self.lib.nsg_incr_last_error(h).decode() if h.value else "nsg_incr_create failed"
+            if h.value:
+                self.lib.nsg_incr_destroy(h)
+                self._h = ctypes.c_void_p()
+            raise NSGError(rc, msg)
+        self.N = int(N)
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise NSGError(rc, self.lib.nsg_incr_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self.lib.nsg_incr_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def increments(self) -> np.ndarray:
+        r = np.zeros(self.NUM, dtype=np.int32)
+        self._check(self.lib.nsg_incr_increments(self._h, _ptr(r)))
+        return r
+
+    def _field(self, u):
+        a = np.ascontiguousarray(u, dtype=np.float64)
+        if a.shape != (self.N,) * 3 + (3,):
+            raise ValueError(f"field shape {a.shape} != {(self.N,) * 3 + (3,)}")
+        return a
+
+    def precompute(self, snap, scale: float = 1.0):
+        if isinstance(snap, Solver):
+            self._check(self.lib.nsg_incr_precompute_state(self._h, snap._h, float(scale)))
+        else:
+            self._check(self.lib.nsg_incr_precompute(self._h, _ptr(self._field(snap))))
+
+    def set_ranges(self):
+        self._check(self.lib.nsg_incr_set_ranges(self._h))
+
+    def histogram(self, snap, scale: float = 1.0):
+        if isinstance(snap, Solver):
+            self._check(self.lib.nsg_incr_histogram_state(self._h, snap._h, float(scale)))
+        else:
+            self._check(self.lib.nsg_incr_histogram(self._h, _ptr(self._field(snap))))
+
+    def results(self, ranges: bool = True):
+        """(stats [9, 6] = min, max, mean, sd, skew, kurtosis; ranges [9, 1001]; counts [9, 1000])"""
+        st = np.empty((self.NUM, 6))
+        rg = np.empty((self.NUM, self.BINS + 1)) if ranges else None
+        ct = np.empty((self.NUM, self.BINS))
+        self._check(self.lib.nsg_incr_results(self._h, _ptr(st), _ptr(rg) if ranges else None, _ptr(ct)))
+        return st, rg, ct
+
+    def reset(self):
+        self._check(self.lib.nsg_incr_reset(self._h))
+
+    def time_pass(self, histogram: bool = False, iters: int = 5) -> float:
+        ms = ctypes.c_double()
+        self._check(self.lib.nsg_incr_time_pass(self._h, int(histogram), int(iters), ctypes.byref(ms)))
         return ms.value

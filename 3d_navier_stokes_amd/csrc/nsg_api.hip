@@ -673,6 +673,14 @@ template <int N> static int realspace_N(nsg_ctx* c, int b, int which) {
                                                1.0 / N3, 0);
 }
 
+int nsgi_realspace(nsg_ctx* c, const double** field) {
+    HIPCHK(c, hipSetDevice(c->p.device));
+    const int rc = dispatch_N(c->N, [&](auto n) { return realspace_N<decltype(n)::value>(c, 0, 0); });
+    if (rc) return rc;
+    *field = (const double*)c->B1;
+    return NSG_OK;
+}
+
 static int realspace_box(nsg_ctx* c, int b, int which, double* out) {
     const int rc = dispatch_N(c->N, [&](auto n) { return realspace_N<decltype(n)::value>(c, b, which); });
     if (rc) return rc;

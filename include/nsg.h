@@ -213,6 +213,47 @@ int nsg_resample(nsg_ctx* src, nsg_ctx* dst);
 int nsg_attach_closure(nsg_ctx* ctx, nsg_cnn* net);
 int nsg_closure_term(nsg_ctx* ctx, double* out);
 
+/* ---------------------------------------------------------------------------------------------
+ * Velocity-increment statistics (the reference's post-processing, PostProcessing/src/stats.c).
+ * Longitudinal increments u_x(i+r,j,k) - u_x(i,j,k), u_y(i,j+r,k) - u_y(i,j,k),
+ * u_z(i,j,k+r) - u_z(i,j,k), shifted index inside the box (no periodic wrap), all three into one
+ * accumulator per separation r in {1,2,4,8,16,32,64,128,N/2} (AllocateStatsObjects, :404-413;
+ * NUM_INCR 9, N_BINS 1000, PostProcessing/src/data_types.h:49-51).  Real fields are AoS
+ * [N][N][N][3] doubles (the post-processing's run_data->u).
+ * ------------------------------------------------------------------------------------------- */
+#define NSG_INCR_NUM 9
+#define NSG_INCR_BINS 1000
+typedef struct nsg_incr nsg_incr;
+/* gsl_rstat_alloc / gsl_histogram_alloc per separation (stats.c:404-423) */
+int nsg_incr_create(int N, int device, nsg_incr** out);
+void nsg_incr_destroy(nsg_incr* s);
+const char* nsg_incr_last_error(const nsg_incr* s);
+/* the separations r[NSG_INCR_NUM] */
+int nsg_incr_increments(const nsg_incr* s, int* r);
+/* Precompute's pass over one snapshot (stats.c:62-112): count, min, max, mean and central moments
+ * of every separation's increments, accumulated over calls.  _state: the snapshot is the
+ * context's real-space velocity (1/N^3-normalised, as nsg_realspace) times `scale`; scale = N^3
+ * gives the post-processing's unnormalised c2r (PostProcessing/src/hdf5_func.c:306) */
+int nsg_incr_precompute(nsg_incr* s, const double* u);
+int nsg_incr_precompute_state(nsg_incr* s, nsg_ctx* ctx, double scale);
+/* end of Precompute (stats.c:118-132): bin ranges [min - 0.05, max + 0.05], uniform as
+ * gsl_histogram_set_ranges_uniform; clears the counts.  NSG_ERR_NONFINITE for NaN/Inf input */
+int nsg_incr_set_ranges(nsg_incr* s);
+/* ComputeStats (stats.c:153-250): histogram counts of one snapshot, accumulated over calls.  An
+ * increment outside the ranges (where the reference exits, :191-195) gives NSG_ERR_INVALID; the
+ * in-range increments of that snapshot stay counted */
+int nsg_incr_histogram(nsg_incr* s, const double* u);
+int nsg_incr_histogram_state(nsg_incr* s, nsg_ctx* ctx, double scale);
+/* the arrays WriteStatsData writes (stats.c:316-371), any may be NULL: stats [9][6] = min, max,
+ * mean, sd, skew, kurtosis (gsl_rstat_*: sd with n - 1, skew and kurtosis over that sd, 0 for an
+ * empty set), ranges [9][1001], counts [9][1000] */
+int nsg_incr_results(nsg_incr* s, double* stats, double* ranges, double* counts);
+/* zero the accumulators and forget the ranges (no reference counterpart) */
+int nsg_incr_reset(nsg_incr* s);
+/* mean time (ms, HIP events) of one moments (hist 0) or histogram (hist 1) pass over the last
+ * host snapshot; the passes accumulate (no reference counterpart) */
+int nsg_incr_time_pass(nsg_incr* s, int hist, int iters, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

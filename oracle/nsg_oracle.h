@@ -79,6 +79,16 @@ void oracle_ic_k4(int N, double k0, double energy, uint64_t seed, double* uh);
 
 void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 
+/* post-processing increment statistics (stats_oracle.c, PostProcessing/src/stats.c) */
+#define ORACLE_NUM_INCR 9
+#define ORACLE_N_BINS 1000
+void oracle_incr_list(int N, int* r);
+/* acc [9][7] = {n, min, max, mean, M2, M3, M4} per increment, updated in the reference's order */
+void oracle_incr_moments(int N, const double* u, double* acc);
+void oracle_incr_ranges(const double* acc, double* ranges);
+long oracle_incr_histogram(int N, const double* u, const double* ranges, double* counts);
+void oracle_incr_stats(const double* acc, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -54,6 +54,12 @@ def load():
     lib.oracle_dp5_factor.restype = d
     lib.oracle_dp5_solve.argtypes = [i, vp, d, i, vp, vp, i, d, d, vp, vp]
     lib.oracle_dp5_solve.restype = i
+    lib.oracle_incr_list.argtypes = [i, vp]
+    lib.oracle_incr_moments.argtypes = [i, vp, vp]
+    lib.oracle_incr_ranges.argtypes = [vp, vp]
+    lib.oracle_incr_histogram.argtypes = [i, vp, vp, vp]
+    lib.oracle_incr_histogram.restype = ctypes.c_long
+    lib.oracle_incr_stats.argtypes = [vp, vp]
     _lib = lib
     return lib
 
@@ -209,3 +215,44 @@ def n_spect(N):
 
 def set_threads(n):
     load().oracle_set_threads(int(n))
+
+
+# ---------------------------------------------------------------- increment statistics
+NUM_INCR, N_BINS = 9, 1000
+
+
+def incr_list(N):
+    r = np.zeros(NUM_INCR, dtype=np.int32)
+    load().oracle_incr_list(int(N), _p(r))
+    return r
+
+
+def incr_moments(u, acc=None):
+    """one Precompute pass over a real field u (N, N, N, 3); acc [9, 7] = n, min, max, mean, M2, M3, M4"""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    acc = np.zeros((NUM_INCR, 7)) if acc is None else acc
+    load().oracle_incr_moments(u.shape[0], _p(u), _p(acc))
+    return acc
+
+
+def incr_ranges(acc):
+    acc = np.ascontiguousarray(acc, dtype=np.float64)
+    out = np.empty((NUM_INCR, N_BINS + 1))
+    load().oracle_incr_ranges(_p(acc), _p(out))
+    return out
+
+
+def incr_histogram(u, ranges, counts=None):
+    """one ComputeStats pass: (counts [9, 1000] accumulated, number of values outside the ranges)"""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    ranges = np.ascontiguousarray(ranges, dtype=np.float64)
+    counts = np.zeros((NUM_INCR, N_BINS)) if counts is None else counts
+    out = load().oracle_incr_histogram(u.shape[0], _p(u), _p(ranges), _p(counts))
+    return counts, int(out)
+
+
+def incr_stats(acc):
+    acc = np.ascontiguousarray(acc, dtype=np.float64)
+    out = np.empty((NUM_INCR, 6))
+    load().oracle_incr_stats(_p(acc), _p(out))
+    return out
