@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -37,35 +38,57 @@ constexpr int PSTRIDE = 8;            // doubles per partial: n, K, s1, s2, s3, 
 struct IncrArgs {
     const double* u;                 // AoS [N][N][N][3]
     double scale;                    // multiplies every value read
-    int N, T, ntiles;
+    int N, T, lgT, ntiles;
     int inc[NINC];
     double* part;                    // moments: [gridDim.x][NINC][PSTRIDE] for this direction
     unsigned long long* counts;      // histogram: [NINC][NBIN]
     unsigned long long* outside;     // histogram: increments outside the ranges
     double lo[NINC], hi[NINC];       // histogram: range[0] and range[NBIN] per separation
+    double inv[NINC];                // histogram: NBIN / (hi - lo), the linear map to bins
+    double eps[NINC];                // histogram: its error bound vs the exact ranges (bin widths)
+    const double* f;                 // histogram: f1[i] = (NBIN-i)/NBIN, f2[i] = i/NBIN (device)
 };
 
-// tile `tile` of direction DIR: LDS line t (0..T-1), point a (0..N-1)
+// tile `tile` of direction DIR: LDS line t (0..T-1), point a (0..N-1).  Loads are issued in
+// batches of LB per thread before their LDS stores, so LB HBM requests per thread are in flight.
+// T is a power of two (lgT = log2 T).
 template <int DIR>
 __device__ __forceinline__ void load_tile(const IncrArgs& A, int tile, double* lds, int LD) {
+    constexpr int LB = 8;
     const long long N = A.N;
-    const int T = A.T;
-    const int per = A.N / T;
-    const long long p0 = tile / per, p1 = (long long)(tile % per) * T;
-    const int n = A.N * T;
-    for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
-        int t, a;
-        long long off;
-        if constexpr (DIR == 2) {  // lines along k: consecutive threads walk the contiguous k
-            a = idx % A.N;
-            t = idx / A.N;
-            off = ((p0 * N + p1 + t) * N + a) * 3 + 2;
-        } else {                   // lines along i (DIR 0, position (j,k)) or j (DIR 1, position (i,k))
-            t = idx % T;
-            a = idx / T;
-            off = DIR == 0 ? (((long long)a * N + p0) * N + p1 + t) * 3 + 0 : ((p0 * N + a) * N + p1 + t) * 3 + 1;
+    const int T = A.T, lgT = A.lgT;
+    const int per = A.N >> lgT;
+    const long long p0 = tile / per, p1 = (long long)(tile % per) << lgT;
+    const int n = A.N << lgT;
+    // DIR 2 walks idx = t N + a (a contiguous in memory): (t, a) advanced without dividing
+    int t2 = 0, a2 = 0;
+    if constexpr (DIR == 2) {
+        t2 = threadIdx.x / A.N;
+        a2 = threadIdx.x - t2 * A.N;
+    }
+    for (int base = 0; base < n; base += LB * TB) {
+        double v[LB];
+        int li[LB];
+#pragma unroll
+        for (int b = 0; b < LB; ++b) {
+            const int idx = base + b * TB + threadIdx.x;
+            long long off = 0;
+            if constexpr (DIR == 2) {
+                off = ((p0 * N + p1 + t2) * N + a2) * 3 + 2;
+                li[b] = t2 * LD + a2;
+                a2 += TB;
+                while (a2 >= A.N) { a2 -= A.N; ++t2; }
+            } else {
+                const int t = idx & (T - 1), a = idx >> lgT;  // lines along i (DIR 0) or j (DIR 1)
+                off = DIR == 0 ? (((long long)a * N + p0) * N + p1 + t) * 3 + 0 : ((p0 * N + a) * N + p1 + t) * 3 + 1;
+                li[b] = t * LD + a;
+            }
+            v[b] = 0.0;
+            if (idx < n) v[b] = A.u[off];  // off is inside the field only for idx < n
         }
-        lds[t * LD + a] = A.scale * A.u[off];
+#pragma unroll
+        for (int b = 0; b < LB; ++b)
+            if (base + b * TB + (int)threadIdx.x < n) lds[li[b]] = A.scale * v[b];
     }
 }
 
@@ -109,22 +132,28 @@ __global__ void __launch_bounds__(TB) incr_moments_kernel(IncrArgs A) {
             for (int q = 0; q < NINC; ++q) K[q] = A.inc[q] < A.N ? lds[A.inc[q]] - lds[0] : 0.0;
             first = false;
         }
+        // point p = t N + a walked with stride TB ((t, a) advanced without dividing); each point
+        // is the left end of its increment at all nine separations
+        int t = threadIdx.x / A.N, a = threadIdx.x - t * A.N;
+        for (; t < A.T;) {
+            const int base = t * LD + a;
+            const double x0 = lds[base];
 #pragma unroll
-        for (int q = 0; q < NINC; ++q) {
-            const int r = A.inc[q];
-            if (r >= A.N) continue;
-            const int len = A.N - r, n = len * A.T;
-            for (int p = threadIdx.x; p < n; p += TB) {
-                const int t = p / len, a = p - t * len;
-                const double d = lds[t * LD + a + r] - lds[t * LD + a];
-                const double y = d - K[q], y2 = y * y;
-                s1[q] += y;
-                s2[q] += y2;
-                s3[q] += y2 * y;
-                s4[q] += y2 * y2;
-                mn[q] = fmin(mn[q], d);
-                mx[q] = fmax(mx[q], d);
+            for (int q = 0; q < NINC; ++q) {
+                const int r = A.inc[q];
+                if (a + r < A.N) {
+                    const double d = lds[base + r] - x0;
+                    const double y = d - K[q], y2 = y * y;
+                    s1[q] += y;
+                    s2[q] += y2;
+                    s3[q] += y2 * y;
+                    s4[q] += y2 * y2;
+                    mn[q] = fmin(mn[q], d);
+                    mx[q] = fmax(mx[q], d);
+                }
             }
+            a += TB;
+            while (a >= A.N) { a -= A.N; ++t; }
         }
     }
     // block reduction, one separation at a time
@@ -218,43 +247,55 @@ __global__ void __launch_bounds__(TB) incr_finalize_kernel(double* acc, const do
 // ------------------------------------------------------------------------------------------
 // histogram pass
 // ------------------------------------------------------------------------------------------
-// range i of the uniform histogram on [lo, hi], n = NBIN, evaluated without contraction
-__device__ __forceinline__ double range_at(int i, double lo, double hi) {
+// range i of the uniform histogram on [lo, hi]: f1[i] lo + f2[i] hi with f1[i] = (n-i)/n and
+// f2[i] = i/n (tables made on the host), evaluated without contraction as the host does
+__device__ __forceinline__ double range_at(const double* f, int i, double lo, double hi) {
 #pragma clang fp contract(off)
-    const double f1 = (double)(NBIN - i) / (double)NBIN;
-    const double f2 = (double)i / (double)NBIN;
-    return f1 * lo + f2 * hi;
+    return f[i] * lo + f[NBIN + 1 + i] * hi;
 }
 
 template <int DIR>
 __global__ void __launch_bounds__(TB) incr_hist_kernel(IncrArgs A) {
     extern __shared__ double lds[];
     const int LD = A.N + 1;
-    unsigned int* hist = (unsigned int*)(lds + (size_t)A.T * LD);  // [NINC][NBIN]
+    double* f = lds + (size_t)A.T * LD;                              // f1 [NBIN+1], f2 [NBIN+1]
+    unsigned int* hist = (unsigned int*)(f + 2 * (NBIN + 1));       // [NINC][NBIN]
+    for (int b = threadIdx.x; b < 2 * (NBIN + 1); b += TB) f[b] = A.f[b];
     for (int b = threadIdx.x; b < NINC * NBIN; b += TB) hist[b] = 0u;
     unsigned long long out = 0;
     for (int tile = blockIdx.x; tile < A.ntiles; tile += gridDim.x) {
         __syncthreads();
         load_tile<DIR>(A, tile, lds, LD);
         __syncthreads();
-        for (int q = 0; q < NINC; ++q) {
-            const int r = A.inc[q];
-            if (r >= A.N) continue;
-            const double lo = A.lo[q], hi = A.hi[q];
-            const int len = A.N - r, n = len * A.T;
-            for (int p = threadIdx.x; p < n; p += TB) {
-                const int t = p / len, a = p - t * len;
-                const double x = lds[t * LD + a + r] - lds[t * LD + a];
+        int t = threadIdx.x / A.N, a = threadIdx.x - t * A.N;
+        for (; t < A.T;) {
+            const int base = t * LD + a;
+            const double x0 = lds[base];
+#pragma unroll
+            for (int q = 0; q < NINC; ++q) {
+                const int r = A.inc[q];
+                if (a + r >= A.N) continue;
+                const double lo = A.lo[q], hi = A.hi[q];
+                const double x = lds[base + r] - x0;
                 if (!(x >= lo && x < hi)) {  // outside [range[0], range[n]) or NaN
                     ++out;
                     continue;
                 }
-                int i = (int)((x - lo) / (hi - lo) * NBIN);
-                i = i < 0 ? 0 : (i > NBIN - 1 ? NBIN - 1 : i);
-                while (i > 0 && x < range_at(i, lo, hi)) --i;
-                while (i < NBIN - 1 && x >= range_at(i + 1, lo, hi)) ++i;
+                // the linear map decides unless x lies within the rounding margin of a bin edge
+                // (margin = a bound on the error of the map vs the exact ranges, in bin widths);
+                // there the exact ranges decide
+                const double u = (x - lo) * A.inv[q];
+                int i = (int)u;
+                const double fr = u - (double)i;
+                if (i > NBIN - 1 || fr < A.eps[q] || fr > 1.0 - A.eps[q]) {
+                    i = i > NBIN - 1 ? NBIN - 1 : i;
+                    while (i > 0 && x < range_at(f, i, lo, hi)) --i;
+                    while (i < NBIN - 1 && x >= range_at(f, i + 1, lo, hi)) ++i;
+                }
                 atomicAdd(&hist[q * NBIN + i], 1u);
             }
+            a += TB;
+            while (a >= A.N) { a -= A.N; ++t; }
         }
     }
     __syncthreads();
@@ -270,7 +311,8 @@ __global__ void __launch_bounds__(TB) incr_hist_kernel(IncrArgs A) {
 // object and C ABI
 // ------------------------------------------------------------------------------------------
 struct nsg_incr {
-    int N = 0, device = 0, T = 1, ncu = 256;
+    int N = 0, device = 0, T = 1, Th = 1, ncu = 256;  // lines per tile: moments, histogram
+    double* ftab = nullptr;                           // f1, f2 of the uniform ranges
     int inc[NINC] = {};
     hipStream_t s = nullptr;
     double* u = nullptr;                   // device copy of a host snapshot
@@ -293,11 +335,20 @@ static int ifail(nsg_incr* s, int code, const char* msg) {
         if (e_ != hipSuccess) return ifail((s), NSG_ERR_DEVICE, (std::string(#expr) + ": " + hipGetErrorString(e_)).c_str()); \
     } while (0)
 
-// lines per tile: a divisor of N, with the tile (T (N+1) doubles) within 33 KB
-static int tile_lines(int N) {
+// lines per tile: a divisor of N, with the tile (T (N+1) doubles) within `kb` KB
+static int tile_lines(int N, int kb) {
     for (int T : {16, 8, 4, 2, 1})
-        if (N % T == 0 && (size_t)T * (N + 1) * sizeof(double) <= 33 * 1024) return T;
+        if (N % T == 0 && (size_t)T * (N + 1) * sizeof(double) <= (size_t)kb * 1024) return T;
     return 1;
+}
+static int ilog2(int T) {
+    int l = 0;
+    while ((1 << l) < T) ++l;
+    return l;
+}
+// histogram kernel LDS: tile, f1/f2 tables, NINC x NBIN counters (two blocks per CU fit)
+static size_t hist_lds_bytes(int N, int Th) {
+    return (size_t)Th * (N + 1) * sizeof(double) + 2 * (NBIN + 1) * sizeof(double) + (size_t)NINC * NBIN * sizeof(unsigned int);
 }
 
 static IncrArgs base_args(const nsg_incr* s, const double* u, double scale) {
@@ -306,6 +357,7 @@ static IncrArgs base_args(const nsg_incr* s, const double* u, double scale) {
     a.scale = scale;
     a.N = s->N;
     a.T = s->T;
+    a.lgT = ilog2(s->T);
     a.ntiles = s->N * (s->N / s->T);
     for (int q = 0; q < NINC; ++q) a.inc[q] = s->inc[q];
     return a;
@@ -332,8 +384,16 @@ static int run_pass(nsg_incr* s, const double* u, double scale, int hist, hipStr
         for (int q = 0; q < NINC; ++q) {
             a.lo[q] = s->ranges[q][0];
             a.hi[q] = s->ranges[q][NBIN];
+            a.inv[q] = NBIN / (a.hi[q] - a.lo[q]);
+            // each range R(i) = f1 lo + f2 hi carries at most 2 eps (|lo| + |hi|) of rounding, the
+            // map (x - lo) inv at most 2 eps NBIN bin widths; four times their sum is the margin
+            a.eps[q] = 16.0 * DBL_EPSILON * NBIN * (1.0 + (std::fabs(a.lo[q]) + std::fabs(a.hi[q])) / (a.hi[q] - a.lo[q]));
         }
-        const size_t bytes = tile_bytes + (size_t)NINC * NBIN * sizeof(unsigned int);
+        a.f = s->ftab;
+        a.T = s->Th;
+        a.lgT = ilog2(s->Th);
+        a.ntiles = s->N * (s->N / s->Th);
+        const size_t bytes = hist_lds_bytes(s->N, s->Th);
         hipLaunchKernelGGL(incr_hist_kernel<0>, dim3(s->grid), dim3(TB), bytes, st, a);
         hipLaunchKernelGGL(incr_hist_kernel<1>, dim3(s->grid), dim3(TB), bytes, st, a);
         hipLaunchKernelGGL(incr_hist_kernel<2>, dim3(s->grid), dim3(TB), bytes, st, a);
@@ -361,7 +421,8 @@ int nsg_incr_create(int N, int device, nsg_incr** out) {
     if (N < 4 || N > 1024) return ifail(s, NSG_ERR_INVALID, "N must be in [4, 1024]");
     s->N = N;
     s->device = device;
-    s->T = tile_lines(N);
+    s->T = tile_lines(N, 33);
+    s->Th = tile_lines(N, 17);
     const int inc[NINC] = {1, 2, 4, 8, 16, 32, 64, 128, N / 2};
     std::copy(inc, inc + NINC, s->inc);
     IHIP(s, hipSetDevice(device));
@@ -370,7 +431,7 @@ int nsg_incr_create(int N, int device, nsg_incr** out) {
     s->ncu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     const int ntiles = N * (N / s->T);
     s->grid = std::max(1, std::min(ntiles, 2 * s->ncu));  // two resident blocks per CU (VGPRs, LDS)
-    const int hist_lds = (int)((size_t)s->T * (N + 1) * sizeof(double) + (size_t)NINC * NBIN * sizeof(unsigned int));
+    const int hist_lds = (int)hist_lds_bytes(N, s->Th);
     IHIP(s, hipFuncSetAttribute((const void*)&incr_hist_kernel<0>, hipFuncAttributeMaxDynamicSharedMemorySize, hist_lds));
     IHIP(s, hipFuncSetAttribute((const void*)&incr_hist_kernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize, hist_lds));
     IHIP(s, hipFuncSetAttribute((const void*)&incr_hist_kernel<2>, hipFuncAttributeMaxDynamicSharedMemorySize, hist_lds));
@@ -378,6 +439,16 @@ int nsg_incr_create(int N, int device, nsg_incr** out) {
     IHIP(s, hipMalloc(&s->part, sizeof(double) * 3 * s->grid * NINC * PSTRIDE));
     IHIP(s, hipMalloc(&s->acc, sizeof(double) * NINC * 8));
     IHIP(s, hipMalloc(&s->counts, sizeof(unsigned long long) * (NINC * NBIN + 1)));
+    {
+        // gsl_histogram_set_ranges_uniform's weights, the same doubles the host ranges use
+        std::vector<double> f(2 * (NBIN + 1));
+        for (int b = 0; b <= NBIN; ++b) {
+            f[b] = (double)(NBIN - b) / (double)NBIN;
+            f[NBIN + 1 + b] = (double)b / (double)NBIN;
+        }
+        IHIP(s, hipMalloc(&s->ftab, sizeof(double) * f.size()));
+        IHIP(s, hipMemcpy(s->ftab, f.data(), sizeof(double) * f.size(), hipMemcpyHostToDevice));
+    }
     IHIP(s, hipMemset(s->acc, 0, sizeof(double) * NINC * 8));
     IHIP(s, hipMemset(s->counts, 0, sizeof(unsigned long long) * (NINC * NBIN + 1)));
     IHIP(s, hipDeviceSynchronize());
@@ -387,7 +458,7 @@ int nsg_incr_create(int N, int device, nsg_incr** out) {
 void nsg_incr_destroy(nsg_incr* s) {
     if (!s) return;
     (void)hipSetDevice(s->device);
-    for (void* p : {(void*)s->u, (void*)s->part, (void*)s->acc, (void*)s->counts})
+    for (void* p : {(void*)s->u, (void*)s->part, (void*)s->acc, (void*)s->counts, (void*)s->ftab})
         if (p) (void)hipFree(p);
     if (s->s) (void)hipStreamDestroy(s->s);
     delete s;

@@ -20,6 +20,10 @@ replay per ensemble step, against the same steps launched box by box.
 stages, fixed dt = the RK4 run's mean CFL step) in FP64 and FP32-bulk, and the adaptive DP5 loop
 (seven stages per try, FP64) from that dt.
 
+"increment_stats" times the post-processing statistics (longitudinal velocity increments at nine
+separations: moments pass and 1000-bin histogram pass) of the bench's 512^3 FP32-bulk final state
+in real space.
+
 "closure_les" times the learned-closure LES on the 64^3 grid (the forced DNS state filtered with
 cutoff 21, the BF16 MFMA network evaluated every step) and reports the ratio of GPU seconds per
 unit of simulated time, DNS over LES, plus the network's forward throughput against the MI355X
@@ -143,6 +147,31 @@ def run_integrators(nsg, N, steps, warmup, dt_cfl):
     return out
 
 
+def run_incr_stats(nsg, u, iters=3):
+    """velocity-increment statistics (the reference's post-processing) of one 512^3 real-space
+    snapshot resident in HBM: moment pass and histogram pass, each three kernels (one per
+    direction) streaming the AoS field, plus a bounded CPU-oracle sample for scale"""
+    N = u.shape[0]
+    with nsg.IncrementStats(N) as st:
+        st.precompute(u)
+        st.set_ranges()
+        st.histogram(u)
+        stats, _, counts = st.results(ranges=False)
+        ms_m = st.time_pass(False, iters)
+        ms_h = st.time_pass(True, iters)
+    pts = float(N) ** 3
+    out = {"N": N, "moments_ms": ms_m, "histogram_ms": ms_h,
+           "field_GB": 24 * pts / 1e9, "moments_field_GBps": 24 * pts / (ms_m * 1e-3) / 1e9,
+           "histogram_field_GBps": 24 * pts / (ms_h * 1e-3) / 1e9,
+           "moments_stream_GBps": 72 * pts / (ms_m * 1e-3) / 1e9,
+           "histogram_stream_GBps": 72 * pts / (ms_h * 1e-3) / 1e9,
+           "increments_per_pass": float(counts.sum()),
+           "r1_stats_min_max_mean_sd_skew_kurt": [float(x) for x in stats[0]]}
+    log(f"[bench] increment stats {N}^3: moments {ms_m:.2f} ms, histogram {ms_h:.2f} ms per snapshot "
+        f"({out['moments_stream_GBps']:.0f} / {out['histogram_stream_GBps']:.0f} GB/s streamed)")
+    return out
+
+
 ENSEMBLE_SIZES = [32] * 32 + [48] * 32 + [64] * 24 + [96] * 8
 
 
@@ -242,9 +271,17 @@ def cpu_baseline(N):
     t0 = time.perf_counter()
     O.rhs(uh, force_eps=0.1, force_kf=2.5)
     el = time.perf_counter() - t0
+    # the post-processing statistics restatement (sequential, one core) on a 64^3 snapshot
+    n_st = 64
+    u = O.realspace(O.ic_k4(n_st, 3.0, 0.5, 5), 0)
+    t1 = time.perf_counter()
+    O.incr_moments(u)
+    el_st = time.perf_counter() - t1
     return {"value": 1.0 / (4.0 * el), "unit": "steps/s", "cores": threads, "kind": "port",
             "sample": f"one nonlinear-term evaluation (9 FP64 3-D real FFTs + products) at {N}^3 took {el:.2f} s; "
-                      f"steps/s = 1/(4 x that), stage combinations not counted"}
+                      f"steps/s = 1/(4 x that), stage combinations not counted",
+            "increment_stats_moments": {"points_per_s": n_st ** 3 / el_st, "cores": 1,
+                                        "sample": f"one moments pass over a {n_st}^3 snapshot took {el_st:.3f} s"}}
 
 
 def main():
@@ -258,11 +295,22 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=512)
     ap.add_argument("--ensemble-steps", type=int, default=100, help="0 skips the 96-box ensemble")
     ap.add_argument("--no-closure", dest="closure", action="store_false", help="skip the closure-LES timing")
+    ap.add_argument("--no-incr-stats", dest="incr_stats", action="store_false",
+                    help="skip the velocity-increment statistics timing")
     ap.add_argument("--no-integrators", dest="integrators", action="store_false",
                     help="skip the RK5-CN / Dormand-Prince timing")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB (HBM counter calibration)")
+    ap.add_argument("--incr-only", type=int, default=0,
+                    help="only time the increment statistics of a k^4 start field of this size")
     args = ap.parse_args()
+    if args.incr_only:
+        nsg = load_nsg()
+        with nsg.Solver(args.incr_only, precision="fp32bulk") as s:
+            s.init("k4", k0=2.0, energy=0.5, seed=2)
+            u = s.realspace(0)
+        print(json.dumps({"increment_stats": run_incr_stats(nsg, u)}))
+        return
     if args.copy_probe_mib:
         import torch
 
@@ -305,6 +353,7 @@ def main():
     state_err = float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64))
     s64.close()
     clos = run_closure(nsg, s32, r32) if args.closure else None
+    incr = run_incr_stats(nsg, s32.realspace(0)) if args.incr_stats else None
     s32.close()
     integ = run_integrators(nsg, N, max(4, args.steps // 2), 2, r64["mean_dt"]) if args.integrators else None
 
@@ -346,6 +395,7 @@ def main():
         "step_bytes_GB": r32["step_bytes_GB"],
         "step_GBps": r32["step_GBps"],
         "integrators": integ,
+        "increment_stats": incr,
         "ensemble_96_boxes": ens,
         "closure_les": clos,
         "cpu_baseline": cpu,

@@ -78,7 +78,9 @@ def test_increment_stats_from_solver_state(nsg, oracle):
     assert np.array_equal(cc.sum(axis=1), ca.sum(axis=1))
 
 
-def test_linear_ramp_closed_form(nsg):
+def test_linear_ramp_closed_form(nsg, oracle):
+    """every increment a r sits on the middle bin edge of its histogram: the exact-range path of
+    the bin search must pick the oracle's bin"""
     N, a = 16, 0.5
     i = np.arange(N, dtype=np.float64)
     u = np.zeros((N, N, N, 3))
@@ -89,7 +91,9 @@ def test_linear_ramp_closed_form(nsg):
         s.precompute(u)
         s.set_ranges()
         s.histogram(u)
-        st, _, ct = s.results()
+        st, rg, ct = s.results()
+    want, out = oracle.incr_histogram(u, rg)
+    assert out == 0 and np.array_equal(ct, want)
     for q, r in enumerate([1, 2, 4, 8, 16, 32, 64, 128, 8]):
         if r >= N:
             assert np.all(st[q] == 0) and ct[q].sum() == 0
