@@ -258,9 +258,8 @@ template <int DIR>
 __global__ void __launch_bounds__(TB) incr_hist_kernel(IncrArgs A) {
     extern __shared__ double lds[];
     const int LD = A.N + 1;
-    double* f = lds + (size_t)A.T * LD;                              // f1 [NBIN+1], f2 [NBIN+1]
-    unsigned int* hist = (unsigned int*)(f + 2 * (NBIN + 1));       // [NINC][NBIN]
-    for (int b = threadIdx.x; b < 2 * (NBIN + 1); b += TB) f[b] = A.f[b];
+    const double* f = A.f;                                     // f1, f2 (global: rarely read)
+    unsigned int* hist = (unsigned int*)(lds + (size_t)A.T * LD);  // [NINC][NBIN]
     for (int b = threadIdx.x; b < NINC * NBIN; b += TB) hist[b] = 0u;
     unsigned long long out = 0;
     for (int tile = blockIdx.x; tile < A.ntiles; tile += gridDim.x) {
@@ -346,9 +345,9 @@ static int ilog2(int T) {
     while ((1 << l) < T) ++l;
     return l;
 }
-// histogram kernel LDS: tile, f1/f2 tables, NINC x NBIN counters (two blocks per CU fit)
+// histogram kernel LDS: tile and NINC x NBIN counters (three blocks per CU fit)
 static size_t hist_lds_bytes(int N, int Th) {
-    return (size_t)Th * (N + 1) * sizeof(double) + 2 * (NBIN + 1) * sizeof(double) + (size_t)NINC * NBIN * sizeof(unsigned int);
+    return (size_t)Th * (N + 1) * sizeof(double) + (size_t)NINC * NBIN * sizeof(unsigned int);
 }
 
 static IncrArgs base_args(const nsg_incr* s, const double* u, double scale) {
@@ -394,9 +393,10 @@ static int run_pass(nsg_incr* s, const double* u, double scale, int hist, hipStr
         a.lgT = ilog2(s->Th);
         a.ntiles = s->N * (s->N / s->Th);
         const size_t bytes = hist_lds_bytes(s->N, s->Th);
-        hipLaunchKernelGGL(incr_hist_kernel<0>, dim3(s->grid), dim3(TB), bytes, st, a);
-        hipLaunchKernelGGL(incr_hist_kernel<1>, dim3(s->grid), dim3(TB), bytes, st, a);
-        hipLaunchKernelGGL(incr_hist_kernel<2>, dim3(s->grid), dim3(TB), bytes, st, a);
+        const int hg = std::max(1, std::min(a.ntiles, 3 * s->ncu));  // three resident blocks per CU
+        hipLaunchKernelGGL(incr_hist_kernel<0>, dim3(hg), dim3(TB), bytes, st, a);
+        hipLaunchKernelGGL(incr_hist_kernel<1>, dim3(hg), dim3(TB), bytes, st, a);
+        hipLaunchKernelGGL(incr_hist_kernel<2>, dim3(hg), dim3(TB), bytes, st, a);
         IHIP(s, hipGetLastError());
     }
     IHIP(s, hipStreamSynchronize(st));
