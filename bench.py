@@ -271,8 +271,8 @@ def cpu_baseline(N):
     t0 = time.perf_counter()
     O.rhs(uh, force_eps=0.1, force_kf=2.5)
     el = time.perf_counter() - t0
-    # the post-processing statistics restatement (sequential, one core) on a 64^3 snapshot
-    n_st = 64
+    # the post-processing statistics restatement (sequential, one core) on a 128^3 snapshot
+    n_st = 128
     u = O.realspace(O.ic_k4(n_st, 3.0, 0.5, 5), 0)
     t1 = time.perf_counter()
     O.incr_moments(u)
