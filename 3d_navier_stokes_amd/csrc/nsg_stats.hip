@@ -404,7 +404,8 @@ static int run_pass(nsg_incr* s, const double* u, double scale, int hist, hipStr
         unsigned long long outside = 0;
         IHIP(s, hipMemcpy(&outside, s->counts + NINC * NBIN, sizeof outside, hipMemcpyDeviceToHost));
         if (outside) {
-            IHIP(s, hipMemset(s->counts + NINC * NBIN, 0, sizeof outside));
+            IHIP(s, hipMemsetAsync(s->counts + NINC * NBIN, 0, sizeof outside, st));
+            IHIP(s, hipStreamSynchronize(st));
             return ifail(s, NSG_ERR_INVALID,
                          (std::to_string(outside) + " increments fall outside the bin ranges (the reference exits here)").c_str());
         }
@@ -518,7 +519,9 @@ int nsg_incr_set_ranges(nsg_incr* s) {
             s->ranges[q][b] = f1 * xmin + f2 * xmax;
         }
     }
-    IHIP(s, hipMemset(s->counts, 0, sizeof(unsigned long long) * (NINC * NBIN + 1)));
+    // the passes run on non-blocking streams: clear on the object's stream and wait
+    IHIP(s, hipMemsetAsync(s->counts, 0, sizeof(unsigned long long) * (NINC * NBIN + 1), s->s));
+    IHIP(s, hipStreamSynchronize(s->s));
     s->have_ranges = true;
     return NSG_OK;
 }
@@ -557,9 +560,9 @@ int nsg_incr_results(nsg_incr* s, double* stats, double* ranges, double* counts)
 int nsg_incr_reset(nsg_incr* s) {
     if (!s) return NSG_ERR_INVALID;
     IHIP(s, hipSetDevice(s->device));
-    IHIP(s, hipMemset(s->acc, 0, sizeof(double) * NINC * 8));
-    IHIP(s, hipMemset(s->counts, 0, sizeof(unsigned long long) * (NINC * NBIN + 1)));
-    IHIP(s, hipDeviceSynchronize());
+    IHIP(s, hipMemsetAsync(s->acc, 0, sizeof(double) * NINC * 8, s->s));
+    IHIP(s, hipMemsetAsync(s->counts, 0, sizeof(unsigned long long) * (NINC * NBIN + 1), s->s));
+    IHIP(s, hipStreamSynchronize(s->s));
     s->have_ranges = false;
     return NSG_OK;
 }
