@@ -52,7 +52,7 @@ struct IncrArgs {
 // tile `tile` of direction DIR: LDS line t (0..T-1), point a (0..N-1).  Loads are issued in
 // batches of LB per thread before their LDS stores, so LB HBM requests per thread are in flight.
 // T is a power of two (lgT = log2 T).
-template <int DIR>
+template <int DIR, int NT>
 __device__ __forceinline__ void load_tile(const IncrArgs& A, int tile, double* lds, int LD) {
     constexpr int LB = 8;
     const long long N = A.N;
@@ -66,17 +66,17 @@ __device__ __forceinline__ void load_tile(const IncrArgs& A, int tile, double* l
         t2 = threadIdx.x / A.N;
         a2 = threadIdx.x - t2 * A.N;
     }
-    for (int base = 0; base < n; base += LB * TB) {
+    for (int base = 0; base < n; base += LB * NT) {
         double v[LB];
         int li[LB];
 #pragma unroll
         for (int b = 0; b < LB; ++b) {
-            const int idx = base + b * TB + threadIdx.x;
+            const int idx = base + b * NT + threadIdx.x;
             long long off = 0;
             if constexpr (DIR == 2) {
                 off = ((p0 * N + p1 + t2) * N + a2) * 3 + 2;
                 li[b] = t2 * LD + a2;
-                a2 += TB;
+                a2 += NT;
                 while (a2 >= A.N) { a2 -= A.N; ++t2; }
             } else {
                 const int t = idx & (T - 1), a = idx >> lgT;  // lines along i (DIR 0) or j (DIR 1)
@@ -88,7 +88,7 @@ __device__ __forceinline__ void load_tile(const IncrArgs& A, int tile, double* l
         }
 #pragma unroll
         for (int b = 0; b < LB; ++b)
-            if (base + b * TB + (int)threadIdx.x < n) lds[li[b]] = A.scale * v[b];
+            if (base + b * NT + (int)threadIdx.x < n) lds[li[b]] = A.scale * v[b];
     }
 }
 
@@ -109,81 +109,60 @@ __device__ __forceinline__ double wave_max(double v) {
 }
 
 // ------------------------------------------------------------------------------------------
-// moments pass
+// moments pass: one wavefront per separation (NINC wavefronts per block), so a thread carries
+// the 7 accumulators of one separation only and many wavefronts fit on a SIMD
 // ------------------------------------------------------------------------------------------
+constexpr int TBM = 64 * NINC;
+
 template <int DIR>
-__global__ void __launch_bounds__(TB) incr_moments_kernel(IncrArgs A) {
+__global__ void __launch_bounds__(TBM) incr_moments_kernel(IncrArgs A) {
     extern __shared__ double lds[];
-    __shared__ double red[TB / 64][6];
     const int LD = A.N + 1;
-    double K[NINC], s1[NINC], s2[NINC], s3[NINC], s4[NINC], mn[NINC], mx[NINC];
-#pragma unroll
-    for (int q = 0; q < NINC; ++q) {
-        K[q] = 0.0; s1[q] = 0.0; s2[q] = 0.0; s3[q] = 0.0; s4[q] = 0.0;
-        mn[q] = INFINITY; mx[q] = -INFINITY;
-    }
+    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int r = A.inc[q];
+    const int len = A.N - r;  // increments per line (<= 0: none)
+    double K = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0, mn = INFINITY, mx = -INFINITY;
     bool first = true;
+    long long tiles_here = 0;
     for (int tile = blockIdx.x; tile < A.ntiles; tile += gridDim.x) {
         __syncthreads();  // the previous tile is consumed
-        load_tile<DIR>(A, tile, lds, LD);
+        load_tile<DIR, TBM>(A, tile, lds, LD);
         __syncthreads();
+        ++tiles_here;
+        if (len <= 0) continue;
         if (first) {
-#pragma unroll
-            for (int q = 0; q < NINC; ++q) K[q] = A.inc[q] < A.N ? lds[A.inc[q]] - lds[0] : 0.0;
+            K = lds[r] - lds[0];  // shift: this block's first increment of the separation
             first = false;
         }
-        // point p = t N + a walked with stride TB ((t, a) advanced without dividing); each point
-        // is the left end of its increment at all nine separations
-        int t = threadIdx.x / A.N, a = threadIdx.x - t * A.N;
-        for (; t < A.T;) {
+        // pair p = t len + a walked with stride 64 ((t, a) advanced without dividing)
+        int t = lane / len, a = lane - t * len;
+        while (t < A.T) {
             const int base = t * LD + a;
-            const double x0 = lds[base];
-#pragma unroll
-            for (int q = 0; q < NINC; ++q) {
-                const int r = A.inc[q];
-                if (a + r < A.N) {
-                    const double d = lds[base + r] - x0;
-                    const double y = d - K[q], y2 = y * y;
-                    s1[q] += y;
-                    s2[q] += y2;
-                    s3[q] += y2 * y;
-                    s4[q] += y2 * y2;
-                    mn[q] = fmin(mn[q], d);
-                    mx[q] = fmax(mx[q], d);
-                }
-            }
-            a += TB;
-            while (a >= A.N) { a -= A.N; ++t; }
+            const double d = lds[base + r] - lds[base];
+            const double y = d - K, y2 = y * y;
+            s1 += y;
+            s2 += y2;
+            s3 += y2 * y;
+            s4 += y2 * y2;
+            mn = fmin(mn, d);
+            mx = fmax(mx, d);
+            a += 64;
+            while (a >= len) { a -= len; ++t; }
         }
     }
-    // block reduction, one separation at a time
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const long long tiles_here = A.ntiles > (int)blockIdx.x ? (A.ntiles - 1 - (int)blockIdx.x) / gridDim.x + 1 : 0;
-#pragma unroll
-    for (int q = 0; q < NINC; ++q) {
-        double v[6] = {wave_sum(s1[q]), wave_sum(s2[q]), wave_sum(s3[q]), wave_sum(s4[q]), wave_min(mn[q]),
-                       wave_max(mx[q])};
-        __syncthreads();
-        if (lane == 0)
-#pragma unroll
-            for (int m = 0; m < 6; ++m) red[wave][m] = v[m];
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double o[6] = {red[0][0], red[0][1], red[0][2], red[0][3], red[0][4], red[0][5]};
-            for (int w = 1; w < TB / 64; ++w) {
-                for (int m = 0; m < 4; ++m) o[m] += red[w][m];
-                o[4] = fmin(o[4], red[w][4]);
-                o[5] = fmax(o[5], red[w][5]);
-            }
-            const int r = A.inc[q];
-            const double cnt = r < A.N ? (double)(A.N - r) * A.T * (double)tiles_here : 0.0;
-            double* dst = A.part + ((long long)blockIdx.x * NINC + q) * PSTRIDE;
-            dst[0] = cnt;
-            dst[1] = K[q];
-            for (int m = 0; m < 4; ++m) dst[2 + m] = o[m];
-            dst[6] = cnt > 0.0 ? o[4] : 0.0;
-            dst[7] = cnt > 0.0 ? o[5] : 0.0;
-        }
+    const double v1 = wave_sum(s1), v2 = wave_sum(s2), v3 = wave_sum(s3), v4 = wave_sum(s4);
+    const double vmn = wave_min(mn), vmx = wave_max(mx);
+    if (lane == 0) {
+        const double cnt = len > 0 ? (double)len * A.T * (double)tiles_here : 0.0;
+        double* dst = A.part + ((long long)blockIdx.x * NINC + q) * PSTRIDE;
+        dst[0] = cnt;
+        dst[1] = K;
+        dst[2] = v1;
+        dst[3] = v2;
+        dst[4] = v3;
+        dst[5] = v4;
+        dst[6] = cnt > 0.0 ? vmn : 0.0;
+        dst[7] = cnt > 0.0 ? vmx : 0.0;
     }
 }
 
@@ -264,7 +243,7 @@ __global__ void __launch_bounds__(TB) incr_hist_kernel(IncrArgs A) {
     unsigned long long out = 0;
     for (int tile = blockIdx.x; tile < A.ntiles; tile += gridDim.x) {
         __syncthreads();
-        load_tile<DIR>(A, tile, lds, LD);
+        load_tile<DIR, TB>(A, tile, lds, LD);
         __syncthreads();
         int t = threadIdx.x / A.N, a = threadIdx.x - t * A.N;
         for (; t < A.T;) {
@@ -369,9 +348,9 @@ static int run_pass(nsg_incr* s, const double* u, double scale, int hist, hipStr
     if (!hist) {
         for (int dir = 0; dir < 3; ++dir) {
             a.part = s->part + (size_t)dir * s->grid * NINC * PSTRIDE;
-            if (dir == 0) hipLaunchKernelGGL(incr_moments_kernel<0>, dim3(s->grid), dim3(TB), tile_bytes, st, a);
-            if (dir == 1) hipLaunchKernelGGL(incr_moments_kernel<1>, dim3(s->grid), dim3(TB), tile_bytes, st, a);
-            if (dir == 2) hipLaunchKernelGGL(incr_moments_kernel<2>, dim3(s->grid), dim3(TB), tile_bytes, st, a);
+            if (dir == 0) hipLaunchKernelGGL(incr_moments_kernel<0>, dim3(s->grid), dim3(TBM), tile_bytes, st, a);
+            if (dir == 1) hipLaunchKernelGGL(incr_moments_kernel<1>, dim3(s->grid), dim3(TBM), tile_bytes, st, a);
+            if (dir == 2) hipLaunchKernelGGL(incr_moments_kernel<2>, dim3(s->grid), dim3(TBM), tile_bytes, st, a);
             IHIP(s, hipGetLastError());
         }
         hipLaunchKernelGGL(incr_finalize_kernel, dim3(NINC), dim3(TB), 0, st, s->acc, (const double*)s->part, 3 * s->grid);
@@ -431,7 +410,7 @@ int nsg_incr_create(int N, int device, nsg_incr** out) {
     IHIP(s, hipGetDeviceProperties(&prop, device));
     s->ncu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     const int ntiles = N * (N / s->T);
-    s->grid = std::max(1, std::min(ntiles, 2 * s->ncu));  // two resident blocks per CU (VGPRs, LDS)
+    s->grid = std::max(1, std::min(ntiles, 3 * s->ncu));  // three resident moment blocks per CU (27 waves)
     const int hist_lds = (int)hist_lds_bytes(N, s->Th);
     IHIP(s, hipFuncSetAttribute((const void*)&incr_hist_kernel<0>, hipFuncAttributeMaxDynamicSharedMemorySize, hist_lds));
     IHIP(s, hipFuncSetAttribute((const void*)&incr_hist_kernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize, hist_lds));
