@@ -233,51 +233,50 @@ __device__ __forceinline__ double range_at(const double* f, int i, double lo, do
     return f[i] * lo + f[NBIN + 1 + i] * hi;
 }
 
+// one wavefront per separation, as the moments pass
 template <int DIR>
-__global__ void __launch_bounds__(TB) incr_hist_kernel(IncrArgs A) {
+__global__ void __launch_bounds__(TBM) incr_hist_kernel(IncrArgs A) {
     extern __shared__ double lds[];
     const int LD = A.N + 1;
     const double* f = A.f;                                     // f1, f2 (global: rarely read)
     unsigned int* hist = (unsigned int*)(lds + (size_t)A.T * LD);  // [NINC][NBIN]
-    for (int b = threadIdx.x; b < NINC * NBIN; b += TB) hist[b] = 0u;
+    for (int b = threadIdx.x; b < NINC * NBIN; b += TBM) hist[b] = 0u;
+    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int r = A.inc[q], len = A.N - r;
+    const double lo = A.lo[q], hi = A.hi[q], inv = A.inv[q], eps = A.eps[q];
+    unsigned int* hq = hist + q * NBIN;
     unsigned long long out = 0;
     for (int tile = blockIdx.x; tile < A.ntiles; tile += gridDim.x) {
         __syncthreads();
-        load_tile<DIR, TB>(A, tile, lds, LD);
+        load_tile<DIR, TBM>(A, tile, lds, LD);
         __syncthreads();
-        int t = threadIdx.x / A.N, a = threadIdx.x - t * A.N;
-        for (; t < A.T;) {
+        if (len <= 0) continue;
+        int t = lane / len, a = lane - t * len;
+        while (t < A.T) {
             const int base = t * LD + a;
-            const double x0 = lds[base];
-#pragma unroll
-            for (int q = 0; q < NINC; ++q) {
-                const int r = A.inc[q];
-                if (a + r >= A.N) continue;
-                const double lo = A.lo[q], hi = A.hi[q];
-                const double x = lds[base + r] - x0;
-                if (!(x >= lo && x < hi)) {  // outside [range[0], range[n]) or NaN
-                    ++out;
-                    continue;
-                }
-                // the linear map decides unless x lies within the rounding margin of a bin edge
-                // (margin = a bound on the error of the map vs the exact ranges, in bin widths);
-                // there the exact ranges decide
-                const double u = (x - lo) * A.inv[q];
-                int i = (int)u;
-                const double fr = u - (double)i;
-                if (i > NBIN - 1 || fr < A.eps[q] || fr > 1.0 - A.eps[q]) {
-                    i = i > NBIN - 1 ? NBIN - 1 : i;
-                    while (i > 0 && x < range_at(f, i, lo, hi)) --i;
-                    while (i < NBIN - 1 && x >= range_at(f, i + 1, lo, hi)) ++i;
-                }
-                atomicAdd(&hist[q * NBIN + i], 1u);
+            const double x = lds[base + r] - lds[base];
+            a += 64;
+            while (a >= len) { a -= len; ++t; }
+            if (!(x >= lo && x < hi)) {  // outside [range[0], range[n]) or NaN
+                ++out;
+                continue;
             }
-            a += TB;
-            while (a >= A.N) { a -= A.N; ++t; }
+            // the linear map decides unless x lies within the rounding margin of a bin edge
+            // (margin = a bound on the error of the map vs the exact ranges, in bin widths); there
+            // the exact ranges decide
+            const double u = (x - lo) * inv;
+            int i = (int)u;
+            const double fr = u - (double)i;
+            if (i > NBIN - 1 || fr < eps || fr > 1.0 - eps) {
+                i = i > NBIN - 1 ? NBIN - 1 : i;
+                while (i > 0 && x < range_at(f, i, lo, hi)) --i;
+                while (i < NBIN - 1 && x >= range_at(f, i + 1, lo, hi)) ++i;
+            }
+            atomicAdd(&hq[i], 1u);
         }
     }
     __syncthreads();
-    for (int b = threadIdx.x; b < NINC * NBIN; b += TB)
+    for (int b = threadIdx.x; b < NINC * NBIN; b += TBM)
         if (hist[b]) atomicAdd(&A.counts[b], (unsigned long long)hist[b]);
     double o = wave_sum((double)out);
     if ((threadIdx.x & 63) == 0 && o > 0.0) atomicAdd(A.outside, (unsigned long long)o);
@@ -373,9 +372,9 @@ static int run_pass(nsg_incr* s, const double* u, double scale, int hist, hipStr
         a.ntiles = s->N * (s->N / s->Th);
         const size_t bytes = hist_lds_bytes(s->N, s->Th);
         const int hg = std::max(1, std::min(a.ntiles, 3 * s->ncu));  // three resident blocks per CU
-        hipLaunchKernelGGL(incr_hist_kernel<0>, dim3(hg), dim3(TB), bytes, st, a);
-        hipLaunchKernelGGL(incr_hist_kernel<1>, dim3(hg), dim3(TB), bytes, st, a);
-        hipLaunchKernelGGL(incr_hist_kernel<2>, dim3(hg), dim3(TB), bytes, st, a);
+        hipLaunchKernelGGL(incr_hist_kernel<0>, dim3(hg), dim3(TBM), bytes, st, a);
+        hipLaunchKernelGGL(incr_hist_kernel<1>, dim3(hg), dim3(TBM), bytes, st, a);
+        hipLaunchKernelGGL(incr_hist_kernel<2>, dim3(hg), dim3(TBM), bytes, st, a);
         IHIP(s, hipGetLastError());
     }
     IHIP(s, hipStreamSynchronize(st));
