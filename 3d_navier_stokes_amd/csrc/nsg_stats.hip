@@ -371,7 +371,9 @@ static int run_pass(nsg_incr* s, const double* u, double scale, int hist, hipStr
         a.lgT = ilog2(s->Th);
         a.ntiles = s->N * (s->N / s->Th);
         const size_t bytes = hist_lds_bytes(s->N, s->Th);
-        const int hg = std::max(1, std::min(a.ntiles, 3 * s->ncu));  // three resident blocks per CU
+        // resident blocks per CU: as many as the 160 KB LDS holds (two with 8-line tiles at 512^3)
+        const int per_cu = std::max(1, (int)((160 * 1024) / hist_lds_bytes(s->N, s->Th)));
+        const int hg = std::max(1, std::min(a.ntiles, std::min(3, per_cu) * s->ncu));
         hipLaunchKernelGGL(incr_hist_kernel<0>, dim3(hg), dim3(TBM), bytes, st, a);
         hipLaunchKernelGGL(incr_hist_kernel<1>, dim3(hg), dim3(TBM), bytes, st, a);
         hipLaunchKernelGGL(incr_hist_kernel<2>, dim3(hg), dim3(TBM), bytes, st, a);
@@ -401,7 +403,7 @@ int nsg_incr_create(int N, int device, nsg_incr** out) {
     s->N = N;
     s->device = device;
     s->T = tile_lines(N, 33);
-    s->Th = tile_lines(N, 17);
+    s->Th = tile_lines(N, 33);
     const int inc[NINC] = {1, 2, 4, 8, 16, 32, 64, 128, N / 2};
     std::copy(inc, inc + NINC, s->inc);
     IHIP(s, hipSetDevice(device));
