@@ -32,6 +32,15 @@ def test_cli_rejects_bad_arguments(solver_bin, args, msg):
 def test_cli_help(solver_bin):
     p = subprocess.run([solver_bin, "--help"], capture_output=True, text=True)
     assert p.returncode == 0 and "-n N" in p.stderr
+    assert "rk5cn" in p.stderr and "dp5" in p.stderr
+
+
+@pytest.mark.parametrize("extra,msg", [(["--adaptive"], "cfl must be 0"), (["--precision", "fp32bulk"], "FP32 round-off")])
+def test_cli_dormand_prince_combinations_rejected(solver_bin, tmp_path, extra, msg):
+    """the adaptive scheme chooses dt itself and needs FP64: refused before any device call"""
+    p = subprocess.run([solver_bin, "-n", "16", "-n", "16", "-n", "16", "--scheme", "dp5", "-o", str(tmp_path) + "/"]
+                       + extra, capture_output=True, text=True)
+    assert p.returncode == 1 and msg in p.stderr + p.stdout
 
 
 @pytest.fixture(scope="module")
