@@ -276,13 +276,13 @@ static int launch_passz_rhs(nsg_ctx* c, int b0, int nb, double scale, int track_
 // strides in elements of the input/output type
 template <typename T, int N, int S, int MODE>
 static int launch_passz(nsg_ctx* c, const void* in, void* out, long long ibox, long long obox, int nb, DevScalars* sc,
-                        double scale, int track_max) {
+                        double scale, int track_max, int soa = 0) {
     constexpr int EPTZ = EZof<T, N, S>(), TPFZ = N / 2 / EPTZ;
     constexpr int LPB = (256 / TPFZ) > 0 ? (256 / TPFZ) : 1;
     PZArgs a;
     a.in = in; a.out = out; a.ibox = ibox; a.obox = obox; a.nbox = nb;
     a.twM = twM_of<T>(c); a.twN = twN_of<T>(c); a.sc = sc;
-    a.scale = scale; a.kmax = c->kmax; a.NZP = c->NZP; a.track_max = track_max;
+    a.scale = scale; a.kmax = c->kmax; a.NZP = c->NZP; a.track_max = track_max; a.soa = soa;
     const long long lines = (long long)N * N * nb;
     const unsigned blocks = (unsigned)((lines + LPB - 1) / LPB);
     prof_begin(c);
@@ -662,20 +662,21 @@ static int measures_box(nsg_ctx* c, int b, nsg_totals* out, double* E_k, double*
     return NSG_OK;
 }
 
-// real-space u (which 0) or w (which 1) of box b as AoS [N^3][3] doubles at the start of B1
-template <int N> static int realspace_N(nsg_ctx* c, int b, int which) {
+// real-space u (which 0) or w (which 1) of box b as AoS [N^3][3] doubles (soa: three planes
+// [3][N^3]) at the start of B1
+template <int N> static int realspace_N(nsg_ctx* c, int b, int which, int soa = 0) {
     int rc;
     if ((rc = launch_pass1<double, N, 0>(c, c->uh, b, 1, nullptr, 1, which == 0 ? 1 : 2, 0))) return rc;
     // pass 2 on three fields: A1 [3][N][NK][NZP] -> A2 [3][N][N][NZP]
     if ((rc = launch_passy<double, N, 0, +1>(c, b, 1, 3))) return rc;
     const double N3 = (double)N * N * N;
     return launch_passz<double, N, 0, ZM_REAL>(c, (const double2*)c->B2 + b * c->a2box, c->B1, 0, 0, 1, c->sc + b,
-                                               1.0 / N3, 0);
+                                               1.0 / N3, 0, soa);
 }
 
-int nsgi_realspace(nsg_ctx* c, const double** field) {
+int nsgi_realspace(nsg_ctx* c, const double** field, int soa) {
     HIPCHK(c, hipSetDevice(c->p.device));
-    const int rc = dispatch_N(c->N, [&](auto n) { return realspace_N<decltype(n)::value>(c, 0, 0); });
+    const int rc = dispatch_N(c->N, [&](auto n) { return realspace_N<decltype(n)::value>(c, 0, 0, soa); });
     if (rc) return rc;
     *field = (const double*)c->B1;
     return NSG_OK;

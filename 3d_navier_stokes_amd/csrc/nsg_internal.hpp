@@ -89,9 +89,10 @@ static int fail(nsg_ctx* c, int code, const char* fmt, ...) {
 int nsgi_c2r(nsg_ctx* c, const double2* src, double* dst);
 // r2c of a physical real AoS field [N^3][3] at src (outside B1) into compact FP64 dst, 2/3 mask applied
 int nsgi_r2c(nsg_ctx* c, const double* src, double2* dst);
-// real-space velocity of box 0 (1/N^3-normalised c2r, AoS [N^3][3]) left in the context's
-// scratch B1; *field points at it until the context's next operation
-int nsgi_realspace(nsg_ctx* c, const double** field);
+// real-space velocity of box 0 (1/N^3-normalised c2r; AoS [N^3][3], or three planes [3][N^3]
+// when soa) left in the context's scratch B1; *field points at it until the context's next
+// operation
+int nsgi_realspace(nsg_ctx* c, const double** field, int soa);
 // implemented in nsg_closure.hip: evaluate the closure term of box 0's state into c->closure
 int nsgi_closure_eval(nsg_ctx* c);
 void nsgi_closure_free(nsg_ctx* c);

@@ -349,6 +349,7 @@ struct PZArgs {
     int nbox;
     int kmax, NZP;
     int track_max;
+    int soa;  // ZM_REAL only: 1 writes the components as three planes [3][x][y][z]
 };
 
 template <int M, int EPTZ> constexpr int zsync() { return (M / EPTZ <= 64 && 64 % (M / EPTZ) == 0) ? SYNC_WAVE : SYNC_BLOCK; }
@@ -472,6 +473,8 @@ __global__ void __launch_bounds__(N / 2 / EPTZ * LPB) passz_kernel(PZArgs a) {
     } else if constexpr (MODE == ZM_REAL) {
         const C* in = (const C*)a.in + box * a.ibox + lid * NZP;
         double* out = (double*)a.out + box * a.obox + lid * (long long)N * 3;
+        const long long plane3 = (long long)N * N * N;
+        double* outs = (double*)a.out + box * a.obox + lid * (long long)N;  // SoA: line lid of plane c
         for (int c = 0; c < 3; ++c) {
             C v[EPTZ];
             z_c2r_field<T, N, EPTZ>(v, in + c * fstride, kmax, t, lds, base, twM, twN);
@@ -479,8 +482,12 @@ __global__ void __launch_bounds__(N / 2 / EPTZ * LPB) passz_kernel(PZArgs a) {
 #pragma unroll
                 for (int m = 0; m < EPTZ; ++m) {
                     const int n = t + TPF * m;
-                    out[(2 * n) * 3 + c] = (double)v[m].x * a.scale;
-                    out[(2 * n + 1) * 3 + c] = (double)v[m].y * a.scale;
+                    if (a.soa) {
+                        ((double2*)(outs + c * plane3))[n] = make_double2((double)v[m].x * a.scale, (double)v[m].y * a.scale);
+                    } else {
+                        out[(2 * n) * 3 + c] = (double)v[m].x * a.scale;
+                        out[(2 * n + 1) * 3 + c] = (double)v[m].y * a.scale;
+                    }
                 }
             }
         }

@@ -36,7 +36,8 @@ constexpr int TB = 256;               // threads per block
 constexpr int PSTRIDE = 8;            // doubles per partial: n, K, s1, s2, s3, s4, min, max
 
 struct IncrArgs {
-    const double* u;                 // AoS [N][N][N][3]
+    const double* u;                 // AoS [N][N][N][3], or three planes [3][N][N][N] (soa)
+    int soa;
     double scale;                    // multiplies every value read
     int N, T, lgT, ntiles;
     int inc[NINC];
@@ -73,16 +74,19 @@ __device__ __forceinline__ void load_tile(const IncrArgs& A, int tile, double* l
         for (int b = 0; b < LB; ++b) {
             const int idx = base + b * NT + threadIdx.x;
             long long off = 0;
+            // point (i, j, k) of component DIR: AoS 3 ((i N + j) N + k) + DIR, SoA DIR N^3 + (i N + j) N + k
+            long long pt;
             if constexpr (DIR == 2) {
-                off = ((p0 * N + p1 + t2) * N + a2) * 3 + 2;
+                pt = (p0 * N + p1 + t2) * N + a2;
                 li[b] = t2 * LD + a2;
                 a2 += NT;
                 while (a2 >= A.N) { a2 -= A.N; ++t2; }
             } else {
                 const int t = idx & (T - 1), a = idx >> lgT;  // lines along i (DIR 0) or j (DIR 1)
-                off = DIR == 0 ? (((long long)a * N + p0) * N + p1 + t) * 3 + 0 : ((p0 * N + a) * N + p1 + t) * 3 + 1;
+                pt = DIR == 0 ? ((long long)a * N + p0) * N + p1 + t : (p0 * N + a) * N + p1 + t;
                 li[b] = t * LD + a;
             }
+            off = A.soa ? (long long)DIR * N * N * N + pt : pt * 3 + DIR;
             v[b] = 0.0;
             if (idx < n) v[b] = A.u[off];  // off is inside the field only for idx < n
         }
@@ -328,9 +332,10 @@ static size_t hist_lds_bytes(int N, int Th) {
     return (size_t)Th * (N + 1) * sizeof(double) + (size_t)NINC * NBIN * sizeof(unsigned int);
 }
 
-static IncrArgs base_args(const nsg_incr* s, const double* u, double scale) {
+static IncrArgs base_args(const nsg_incr* s, const double* u, double scale, int soa) {
     IncrArgs a{};
     a.u = u;
+    a.soa = soa;
     a.scale = scale;
     a.N = s->N;
     a.T = s->T;
@@ -341,8 +346,8 @@ static IncrArgs base_args(const nsg_incr* s, const double* u, double scale) {
 }
 
 // moments or histogram pass over a device field on `st`
-static int run_pass(nsg_incr* s, const double* u, double scale, int hist, hipStream_t st) {
-    IncrArgs a = base_args(s, u, scale);
+static int run_pass(nsg_incr* s, const double* u, double scale, int hist, hipStream_t st, int soa = 0) {
+    IncrArgs a = base_args(s, u, scale, soa);
     const size_t tile_bytes = (size_t)s->T * (s->N + 1) * sizeof(double);
     if (!hist) {
         for (int dir = 0; dir < 3; ++dir) {
@@ -467,9 +472,11 @@ static int state_pass(nsg_incr* s, nsg_ctx* c, double scale, int hist) {
     if (c->nbox != 1 || c->N != s->N || c->p.device != s->device)
         return ifail(s, NSG_ERR_INVALID, "the context must hold one box of the same N on the same device");
     IHIP(s, hipSetDevice(s->device));
+    // the context's c2r writes the field as three planes: each direction's kernel then streams
+    // only its own component
     const double* field = nullptr;
-    if (nsgi_realspace(c, &field) != NSG_OK) return ifail(s, NSG_ERR_DEVICE, c->err.c_str());
-    return run_pass(s, field, scale, hist, c->s);
+    if (nsgi_realspace(c, &field, 1) != NSG_OK) return ifail(s, NSG_ERR_DEVICE, c->err.c_str());
+    return run_pass(s, field, scale, hist, c->s, 1);
 }
 
 int nsg_incr_precompute(nsg_incr* s, const double* u) { return host_pass(s, u, 0); }

@@ -147,7 +147,7 @@ def run_integrators(nsg, N, steps, warmup, dt_cfl):
     return out
 
 
-def run_incr_stats(nsg, u, iters=3):
+def run_incr_stats(nsg, u, iters=3, solver=None):
     """velocity-increment statistics (the reference's post-processing) of one 512^3 real-space
     snapshot resident in HBM: moment pass and histogram pass, each three kernels (one per
     direction) streaming the AoS field, plus a bounded CPU-oracle sample for scale"""
@@ -159,16 +159,36 @@ def run_incr_stats(nsg, u, iters=3):
         stats, _, counts = st.results(ranges=False)
         ms_m = st.time_pass(False, iters)
         ms_h = st.time_pass(True, iters)
+    state = None
+    if solver is not None:
+        # in situ: the solver's c2r writes the field as three planes, each direction's kernel reads
+        # its own component; wall time of the calls (c2r of the state included, one sync each)
+        with nsg.IncrementStats(N) as st:
+            st.precompute(solver)
+            t0 = time.perf_counter()
+            for _ in range(iters):
+                st.precompute(solver)
+            ms_sm = 1e3 * (time.perf_counter() - t0) / iters
+            st.set_ranges()
+            st.histogram(solver)
+            t0 = time.perf_counter()
+            for _ in range(iters):
+                st.histogram(solver)
+            ms_sh = 1e3 * (time.perf_counter() - t0) / iters
+        state = {"moments_ms_incl_c2r": ms_sm, "histogram_ms_incl_c2r": ms_sh}
     pts = float(N) ** 3
     out = {"N": N, "moments_ms": ms_m, "histogram_ms": ms_h,
            "field_GB": 24 * pts / 1e9, "moments_field_GBps": 24 * pts / (ms_m * 1e-3) / 1e9,
            "histogram_field_GBps": 24 * pts / (ms_h * 1e-3) / 1e9,
            "moments_stream_GBps": 72 * pts / (ms_m * 1e-3) / 1e9,
            "histogram_stream_GBps": 72 * pts / (ms_h * 1e-3) / 1e9,
+           "from_solver_state": state,
            "increments_per_pass": float(counts.sum()),
            "r1_stats_min_max_mean_sd_skew_kurt": [float(x) for x in stats[0]]}
     log(f"[bench] increment stats {N}^3: moments {ms_m:.2f} ms, histogram {ms_h:.2f} ms per snapshot "
-        f"({out['moments_stream_GBps']:.0f} / {out['histogram_stream_GBps']:.0f} GB/s streamed)")
+        f"({out['moments_stream_GBps']:.0f} / {out['histogram_stream_GBps']:.0f} GB/s streamed)"
+        + (f"; from the solver state (c2r included) {state['moments_ms_incl_c2r']:.2f} / "
+           f"{state['histogram_ms_incl_c2r']:.2f} ms" if state else ""))
     return out
 
 
@@ -309,7 +329,7 @@ def main():
         with nsg.Solver(args.incr_only, precision="fp32bulk") as s:
             s.init("k4", k0=2.0, energy=0.5, seed=2)
             u = s.realspace(0)
-        print(json.dumps({"increment_stats": run_incr_stats(nsg, u)}))
+            print(json.dumps({"increment_stats": run_incr_stats(nsg, u, solver=s)}))
         return
     if args.copy_probe_mib:
         import torch
@@ -353,7 +373,7 @@ def main():
     state_err = float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64))
     s64.close()
     clos = run_closure(nsg, s32, r32) if args.closure else None
-    incr = run_incr_stats(nsg, s32.realspace(0)) if args.incr_stats else None
+    incr = run_incr_stats(nsg, s32.realspace(0), solver=s32) if args.incr_stats else None
     s32.close()
     integ = run_integrators(nsg, N, max(4, args.steps // 2), 2, r64["mean_dt"]) if args.integrators else None
 

@@ -235,6 +235,7 @@ int nsg_incr_increments(const nsg_incr* s, int* r);
  * context's real-space velocity (1/N^3-normalised, as nsg_realspace) times `scale`; scale = N^3
  * gives the post-processing's unnormalised c2r (PostProcessing/src/hdf5_func.c:306) */
 int nsg_incr_precompute(nsg_incr* s, const double* u);
+/* (the _state calls leave the field in the context's scratch, which its next call reuses) */
 int nsg_incr_precompute_state(nsg_incr* s, nsg_ctx* ctx, double scale);
 /* end of Precompute (stats.c:118-132): bin ranges [min - 0.05, max + 0.05], uniform as
  * gsl_histogram_set_ranges_uniform; clears the counts.  NSG_ERR_NONFINITE for NaN/Inf input */
