@@ -914,11 +914,16 @@ int nsg_profile_passes(nsg_ctx* c, int iters, double* ms, double* bytes) {
     HIPCHK(c, hipSetDevice(c->p.device));
     for (int k = 0; k < 5; ++k) { c->prof_ms[k] = 0.0; c->prof_cnt[k] = 0; }
     HIPCHK(c, hipStreamSynchronize(c->s));
+    // Dormand-Prince tries are not applied here (no accept step): keep the box scalars as they were
+    const bool dp = c->p.scheme == NSG_SCHEME_DP5;
+    std::vector<DevScalars> hs(dp ? c->nbox : 0);
+    if (dp) HIPCHK(c, hipMemcpy(hs.data(), c->sc, sizeof(DevScalars) * c->nbox, hipMemcpyDeviceToHost));
     c->prof = true;
     const int rc = enqueue_steps(c, iters, 0, c->nbox);
     c->prof = false;
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->s));
+    if (dp) HIPCHK(c, hipMemcpy(c->sc, hs.data(), sizeof(DevScalars) * c->nbox, hipMemcpyHostToDevice));
     double tot = 0.0;
     for (int k = 0; k < 5; ++k) {
         ms[k] = c->prof_cnt[k] ? c->prof_ms[k] / c->prof_cnt[k] : 0.0;

@@ -112,12 +112,13 @@ int nsg_realspace(nsg_ctx* ctx, int which, double* out);
 int nsg_get_time(nsg_ctx* ctx, double* t, double* dt, double* max_speed);
 int nsg_set_time(nsg_ctx* ctx, double t, double dt);
 
-/* Per-pass timing of one RK4 step: runs `iters` steps under HIP events on the library's stream and
- * returns the mean duration (ms) of each kernel class: [0] pass1 inverse-x+curl, [1] pass2 inverse-y,
- * [2] pass3 z (c2r, u x w, r2c), [3] pass4 forward-y, [4] pass5 forward-x+epilogue, [5] whole step.
- * bytes[0..4] receive the algorithmic HBM bytes of one launch of each pass (stage 2 form). */
+/* Per-pass timing of one step: runs `iters` steps (DP5: tries, not applied; state and time stay)
+ * under HIP events on the library's stream and returns the mean duration (ms) of each kernel
+ * class: [0] pass1 inverse-x+curl, [1] pass2 inverse-y, [2] pass3 z (c2r, u x w, r2c), [3] pass4
+ * forward-y, [4] pass5 forward-x+epilogue, [5] whole step.  bytes[0..4] receive the algorithmic
+ * HBM bytes of one launch of each pass, averaged over the stages of a step. */
 int nsg_profile_passes(nsg_ctx* ctx, int iters, double* ms, double* bytes);
-/* algorithmic HBM bytes of one full RK4 step (sum over the 20 pass launches) */
+/* algorithmic HBM bytes of one full step (sum over its 5 x stages pass launches) */
 double nsg_step_bytes(const nsg_ctx* ctx);
 
 /* FFT launch schedules.  Every size has schedule 0; N = 32, 48, 64, 96 have alternatives

@@ -168,3 +168,15 @@ def test_dp5_with_zero_closure_is_plain_dp5(nsg, oracle):
         s.set_state(uh)
         s.step(2)
         assert np.array_equal(s.get_state(), want)
+
+
+def test_dp5_profiling_leaves_state_and_time(nsg, oracle):
+    """pass profiling runs tries without the accept step: state, time and dt stay as they were"""
+    N = 32
+    with nsg.Solver(N, nu=0.01, dt=0.02, scheme="dp5") as s:
+        s.set_state(k4(oracle, N))
+        s.step(1)
+        before, tb = s.get_state(), s.time()
+        ms, by = s.profile(2)
+        assert all(m > 0 for m in ms[:5]) and all(b > 0 for b in by)
+        assert np.array_equal(s.get_state(), before) and s.time() == tb
