@@ -1,9 +1,11 @@
-"""FP32-bulk against FP64 on the decaying-turbulence workload, every diagnostic output compared.
+"""FP32-bulk against FP64 on a turbulence workload, every diagnostic output compared.
 
-Workload: periodic [0, 2 pi)^3, N = 256, nu = 2e-3, RK4 (the reference's Crank-Nicolson viscous
-factor), divergence-free random-phase start with E(k) ~ k^4 exp(-2 (k/4)^2), 1/2 <|u|^2> = 0.5,
-Philox seed 1, 500 steps.  Both runs take the same fixed dt (CFL 0.5 of the start field) so that
-they are compared at equal times; both start from the identical FP64 state.  Every `every` steps
+Default workload (decaying): periodic [0, 2 pi)^3, N = 256, nu = 2e-3, RK4 (the reference's
+Crank-Nicolson viscous factor), divergence-free random-phase start with E(k) ~ k^4 exp(-2 (k/4)^2),
+1/2 <|u|^2> = 0.5, Philox seed 1, 500 steps.  With --force-eps the forced workload instead (e.g.
+the 512^3 configuration: --n 512 --nu 1e-3 --k0 2 --seed 2 --force-eps 0.1 --force-kf 2.5).  Both
+runs take the same fixed dt (CFL 0.5 of the start field) so that they are compared at equal
+times; both start from the identical FP64 state.  Every `every` steps
 the totals (energy, enstrophy, palinstrophy, helicity, dissipation) and E(k) of both runs are
 compared; E(k) errors are taken over the shells holding at least 1e-10 of the peak shell's energy
 (below that FP32 round-off of the transforms is an absolute floor); helicity, which is near zero,
@@ -35,18 +37,23 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--every", type=int, default=50)
     ap.add_argument("--cfl", type=float, default=0.5)
+    ap.add_argument("--k0", type=float, default=4.0)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--force-eps", type=float, default=0.0)
+    ap.add_argument("--force-kf", type=float, default=2.5)
     ap.add_argument("--out", default=None)
     a = ap.parse_args()
     nsg = load_nsg()
     N = a.n
-    s64 = nsg.Solver(N, nu=a.nu, dt=1e-3, precision="fp64", cfl=a.cfl)
-    s64.init("k4", k0=4.0, energy=0.5, seed=1)
+    force = dict(force_eps=a.force_eps, force_kf=a.force_kf)
+    s64 = nsg.Solver(N, nu=a.nu, dt=1e-3, precision="fp64", cfl=a.cfl, **force)
+    s64.init("k4", k0=a.k0, energy=0.5, seed=a.seed)
     s64.step(1)  # one CFL step fixes dt from the start field's max|u|
     _, dt, _ = s64.time()
     s64.close()
-    s64 = nsg.Solver(N, nu=a.nu, dt=dt, precision="fp64")
-    s32 = nsg.Solver(N, nu=a.nu, dt=dt, precision="fp32bulk")
-    s64.init("k4", k0=4.0, energy=0.5, seed=1)
+    s64 = nsg.Solver(N, nu=a.nu, dt=dt, precision="fp64", **force)
+    s32 = nsg.Solver(N, nu=a.nu, dt=dt, precision="fp32bulk", **force)
+    s64.init("k4", k0=a.k0, energy=0.5, seed=a.seed)
     s32.set_state(s64.get_state())
     rows = []
     for n in range(0, a.steps + 1, a.every):
@@ -68,8 +75,9 @@ def main():
         rows.append(row)
         log(f"[fp32] step {n}: state {row['state_rel_l2']:.2e}, E(k) {row['E_k_max_rel']:.2e}, "
             f"energy {row['energy_rel']:.2e}, dissipation {row['dissipation_rel']:.2e}")
-    report = {"config": {"N": N, "nu": a.nu, "dt": dt, "steps": a.steps, "ic": "k4 k0=4 seed 1 energy 0.5",
-                         "scheme": "rk4cn"}, "rows": rows}
+    report = {"config": {"N": N, "nu": a.nu, "dt": dt, "steps": a.steps,
+                         "ic": f"k4 k0={a.k0:g} seed {a.seed} energy 0.5", "scheme": "rk4cn",
+                         "force_eps": a.force_eps, "force_kf": a.force_kf}, "rows": rows}
     line = json.dumps(report)
     if a.out:
         with open(a.out, "w") as f:
