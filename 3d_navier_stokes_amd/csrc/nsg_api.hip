@@ -263,10 +263,15 @@ static int launch_passz_rhs(nsg_ctx* c, int b0, int nb, double scale, int track_
     const long long cap = (long long)c->ncu * c->zblocks_per_cu;
     if (blocks > cap) blocks = cap;
     prof_begin(c);
+    // Fields of at most 16 VGPRs per thread: compiled for 2 waves per SIMD.  The kernel's natural
+    // allocation sits at 250-256 VGPRs, and a build one register over would fall to 1 wave per
+    // SIMD (measured 1.7x slower at 512^3 FP32).  Larger fields (FP64, 8 per thread) would spill
+    // under that bound and keep the unbounded build.
+    constexpr int OCC = (EPTZ * sizeof(cpx_t<T>) <= 64) ? 2 : 1;
     if (c->zocc >= 3)
         hipLaunchKernelGGL((passz_rhs_kernel<T, N, EPTZ, WPB, 3>), dim3((unsigned)blocks), dim3(64 * WPB), 0, c->s, a);
     else
-        hipLaunchKernelGGL((passz_rhs_kernel<T, N, EPTZ, WPB, 1>), dim3((unsigned)blocks), dim3(64 * WPB), 0, c->s, a);
+        hipLaunchKernelGGL((passz_rhs_kernel<T, N, EPTZ, WPB, OCC>), dim3((unsigned)blocks), dim3(64 * WPB), 0, c->s, a);
     LAUNCHCHK(c);
     prof_end(c, 2);
     return NSG_OK;
