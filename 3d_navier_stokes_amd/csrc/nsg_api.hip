@@ -36,7 +36,7 @@ using namespace nsg;
 struct SchedV {
     int ex64, ex32, ey64, ey32, ez64, ez32, wx64, wx32, wy64, wy32;
 };
-constexpr int nzp_of(int N) { return ((N / 3 + 1 + 15) / 16) * 16; }
+constexpr int nzp_of(int N) { return ((N / 3 + 1 + 15) / 16) * 16; }  // = Ext<N>::NZP
 constexpr SchedV sched0(int N) {
     //                     ex64 ex32 ey64 ey32 ez64 ez32 wx64 wx32 wy64 wy32
     return N == 8     ? SchedV{8, 8, 8, 8, 4, 4, 8, 16, 8, 16}

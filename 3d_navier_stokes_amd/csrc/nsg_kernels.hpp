@@ -12,6 +12,8 @@
 // Pass 5 performs the forward x transform and the epilogue of NonlinearRHSBatch
 // (Solver/src/solver.c:689-727) fused with the RK stage bookkeeping of RK4Step (:505-608).
 #pragma once
+#include <type_traits>
+
 #include "nsg_fft.hpp"
 
 namespace nsg {
@@ -43,6 +45,14 @@ struct DevScalars {
 // one store instruction)
 constexpr int XPAD = 0;
 __host__ __device__ __forceinline__ long long xplane(int NK, int NZP) { return (long long)NK * NZP + XPAD; }
+// The 2/3 cut kmax = N/3, the compact extent NK = 2 kmax + 1 and the padded kz extent NZP
+// (kmax + 1 rounded up to 16) are functions of N; the context sets the same values.  As
+// compile-time constants they remove the loads, stores and branches of positions that lie outside
+// the cut for every thread: the RHS z pass and the FP32 pass 1 use them (measured faster there;
+// the other passes measured the same or slower and keep the run-time values).
+template <int N> struct Ext {
+    static constexpr int kmax = N / 3, NK = 2 * (N / 3) + 1, NZP = ((N / 3 + 1 + 15) / 16) * 16;
+};
 __device__ __forceinline__ long long sidx(int ix, int iy, int kz, int NK, int NZP) {
     return ((long long)iy * NK + ix) * NZP + kz;
 }
@@ -184,14 +194,17 @@ __global__ void __launch_bounds__(N / EPT * W) pass1_kernel(P1Args a) {
     __shared__ C lds[N * W];
     __shared__ double red[N / EPT * W];
     int tile, iy;
-    if (!tile_of_block<PAIR>(a.NZP / W, a.NK, tile, iy)) return;
+    // compile-time extents: measured faster for the FP32 instance (0.69 -> 0.62 ms at 512^3) and
+    // slower for FP64 (0.94 -> 1.00 ms), so only the FP32 instance takes them
+    constexpr bool CX = !std::is_same<T, double>::value;
+    const int kmax = CX ? Ext<N>::kmax : a.kmax, NK = CX ? Ext<N>::NK : a.NK, NZP = CX ? Ext<N>::NZP : a.NZP;
+    if (!tile_of_block<PAIR>(NZP / W, NK, tile, iy)) return;
     const int box = blockIdx.z;
     const double2* __restrict__ uh = a.uh + box * a.sbox;
     const cpx_t<T>* kprev = a.kprev ? (const cpx_t<T>*)a.kprev + box * a.sbox : nullptr;
     const DevScalars& sc = a.sc[box];
     const int tid = threadIdx.x, w = tid % W, t = tid / W;
     const int kz = tile * W + w;
-    const int kmax = a.kmax, NK = a.NK, NZP = a.NZP;
     const int ky = k_of_cidx(iy, kmax, NK);
     const bool colok = kz <= kmax;
     const double dt = sc.dt, nu = sc.nu;
@@ -598,7 +611,7 @@ __global__ void __launch_bounds__(64 * WPB, OCC) passz_rhs_kernel(PZArgs a) {
 #pragma unroll
     for (int m = 0; m < EPTZ; ++m) ek[m] = ((const C*)a.twN)[t + TPF * m];
     const int base = (wv * LPW + sub) * MP;
-    const int kmax = a.kmax, NZP = a.NZP;
+    constexpr int kmax = Ext<N>::kmax, NZP = Ext<N>::NZP;
     const long long nlines = (long long)N * N;        // lines per box (a multiple of LPW)
     const long long total = nlines * a.nbox;
     const long long ngroups = (total + LPW - 1) / LPW;
