@@ -252,6 +252,7 @@ template <typename T, int N, int S>
 static int launch_passz_rhs(nsg_ctx* c, int b0, int nb, double scale, int track_max) {
     constexpr int EPTZ = EZof<T, N, S>(), TPFZ = N / 2 / EPTZ, LPW = 64 / TPFZ, WPB = 4;
     static_assert(EPTZ >= 1 && LPW >= 1, "bad z configuration");
+    static_assert(Ext<N>::NZP == nzp_of(N), "the kernels' compile-time extents must match the context's");
     PZArgs a;
     a.in = (const char*)c->B2 + b0 * c->a2box * esz<T>();
     a.out = (char*)c->B1 + b0 * c->a3box * esz<T>();
