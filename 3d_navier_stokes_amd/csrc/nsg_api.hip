@@ -627,7 +627,7 @@ static int init_state_box(nsg_ctx* c, int b, int kind, double k0, double energy,
         hipLaunchKernelGGL(ic_k4_kernel, dim3(nb), dim3(256), 0, c->s, uh, c->part, c->N, c->kmax, c->NK, c->NZP, k0,
                            (unsigned long long)seed);
         LAUNCHCHK(c);
-        hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(1), 0, c->s, c->part, (int)nb, c->part + nb);
+        hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(256), 0, c->s, c->part, (int)nb, c->part + nb);
         LAUNCHCHK(c);
         const double N3 = (double)c->N * c->N * c->N;
         hipLaunchKernelGGL(scale_state_kernel, dim3(2048), dim3(256), 0, c->s, uh, 3 * plane, (const double*)(c->part + nb), energy, N3);

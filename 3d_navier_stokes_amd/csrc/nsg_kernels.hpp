@@ -1227,12 +1227,13 @@ __global__ void ic_k4_kernel(double2* __restrict__ uh, double* __restrict__ part
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// one block: thread t sums parts t, t + blockDim, ... in order, then a fixed tree (deterministic)
 __global__ void sum_parts_kernel(const double* __restrict__ part, int n, double* __restrict__ out) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        double s = 0.0;
-        for (int i = 0; i < n; ++i) s += part[i];
-        out[0] = s;
-    }
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += part[i];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) out[0] = s;
 }
 
 __global__ void scale_state_kernel(double2* __restrict__ uh, long long n, const double* __restrict__ esum, double energy, double N3) {
