@@ -642,7 +642,7 @@ static int init_state_box(nsg_ctx* c, int b, int kind, double k0, double energy,
 static int measures_box(nsg_ctx* c, int b, nsg_totals* out, double* E_k, double* Ens_k, int n_spect, int quirk) {
     if ((E_k || Ens_k) && n_spect < c->nspect) return fail(c, NSG_ERR_INVALID, "n_spect %d < %d", n_spect, c->nspect);
     const int ns = c->nspect;
-    hipLaunchKernelGGL(measures_kernel, dim3(ns, 1), dim3(256), 0, c->s, c->uh + b * c->sbox, c->part, c->N, c->kmax,
+    hipLaunchKernelGGL(measures_kernel, dim3(ns, 1), dim3(1024), 0, c->s, c->uh + b * c->sbox, c->part, c->N, c->kmax,
                        c->NK, c->NZP, (const DevScalars*)(c->sc + b), c->p.visc_pow, quirk ? 1 : 0, c->sbox);
     LAUNCHCHK(c);
     std::vector<double> h((size_t)ns * 7);

@@ -1043,10 +1043,12 @@ __device__ __forceinline__ long long isqrt_ll(long long v) {
     return s;
 }
 
-// grid (n_spect, nbox): part[(box * n_spect + b) * 7 + j]
+// grid (n_spect, nbox), block <= 1024 threads (one shell per block: the outer shells hold up to
+// ~4 pi kmax^2 modes, so wide blocks keep the largest shell from setting the time);
+// part[(box * n_spect + b) * 7 + j]
 __global__ void measures_kernel(const double2* __restrict__ uh0, double* __restrict__ part0, int N, int kmax, int NK,
                                 int NZP, const DevScalars* __restrict__ sc, int visc_pow, int quirk, long long sbox) {
-    __shared__ double red[256];
+    __shared__ double red[1024];
     const int b = blockIdx.x, box = blockIdx.y;
     const double2* __restrict__ uh = uh0 + box * sbox;
     double* __restrict__ part = part0 + (long long)box * gridDim.x * 7;
