@@ -554,10 +554,12 @@ __device__ __forceinline__ void zr_c2r(cpx_t<T> (&v)[EPTZ], const cpx_t<T> (&x)[
         C xk = x[m];
         C xr = lds[base + ((M - k) & (M - 1))];  // X[M-k]; for k = 0 this is X[0], replaced below
         if (k == 0) { xk.y = 0; xr = cmk<C>(0, 0); }  // X[N/2] is outside the 2/3 box: zero
-        const C xrc = cconj(xr);
-        const C sm = cadd(xk, xrc), d = csub(xk, xrc);
-        const C e = cconj(ek[m]);  // e^{+2 pi i k / N}
-        v[m] = cadd(sm, cmul_i<+1>(cmul(e, d)));
+        // Z = s + i e^{+2 pi i k/N} d with s = X[k] + X*[M-k], d = X[k] - X*[M-k]; written out so
+        // that each component is two FMAs on s (i conj(ek) = (ek.y, ek.x))
+        using R = decltype(C::x);
+        const R sx = xk.x + xr.x, sy = xk.y - xr.y, dx = xk.x - xr.x, dy = xk.y + xr.y;
+        const C e = ek[m];  // e^{-2 pi i k / N}
+        v[m] = cmk<C>(sx + e.y * dx - e.x * dy, sy + e.y * dy + e.x * dx);
     }
     lds_sync<SY>();
     fft_reg<M, EPTZ, +1, SY, true>(v, t, lds, base, 1, wz);
@@ -580,11 +582,12 @@ __device__ __forceinline__ void zr_r2c_store(cpx_t<T> (&v)[EPTZ], cpx_t<T>* outl
         if (live && k < NZP) {
             C r = cmk<C>(0, 0);
             if (k <= kmax) {
+                // X = (s - i e^{-2 pi i k/N} d) / 2 with s = Z[k] + Z*[M-k], d = Z[k] - Z*[M-k]
                 const C zk = v[m];
-                const C zr = cconj(lds[base + ((M - k) & (M - 1))]);
-                const C sm = cadd(zk, zr), d = csub(zk, zr);
-                const C ed = cmul(ek[m], d);
-                r = cmk<C>(R(0.5) * (sm.x + ed.y), R(0.5) * (sm.y - ed.x));
+                const C zm = lds[base + ((M - k) & (M - 1))];
+                const R sx = zk.x + zm.x, sy = zk.y - zm.y, dx = zk.x - zm.x, dy = zk.y + zm.y;
+                const C e = ek[m];
+                r = cmk<C>(R(0.5) * (sx + e.x * dy + e.y * dx), R(0.5) * (sy - e.x * dx + e.y * dy));
             }
             outline[k] = r;
         }
