@@ -10,6 +10,7 @@
  * The state never leaves the GPU between saves; fixed-dt steps between two saves go to the library
  * as one nsg_step call.
  */
+#define _GNU_SOURCE /* sincos */
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -30,17 +31,19 @@
     } while (0)
 
 /* RANDOM initial condition of the reference (Solver/src/solver.c:1551,1605-1624): srand(123456789),
- * per mode and component amplitude U[0,1] then phase U[0,1] (the C evaluation order inside the
- * reference expression is unspecified; amplitude first is taken here), then the 2/3 mask, which
- * nsg_set_state applies. */
+ * per mode and component amplitude U[0,1] then phase U[0,1] -- the order gcc evaluates the two
+ * rand() calls of the reference's expression in (left factor first) -- and cexp of the phase,
+ * which glibc computes with sincos; then the 2/3 mask, which nsg_set_state applies. */
 static void reference_random_ic(int N, double* uh) {
     const long n = (long)N * N * (N / 2 + 1) * 3;
     srand(123456789);
     for (long i = 0; i < n; ++i) {
         double amp = (double)rand() / (double)RAND_MAX;
         double ph = (double)rand() / (double)RAND_MAX;
-        uh[2 * i] = amp * cos(ph * 2.0 * M_PI);
-        uh[2 * i + 1] = amp * sin(ph * 2.0 * M_PI);
+        double sn, cs;
+        sincos(ph * 2.0 * M_PI, &sn, &cs);
+        uh[2 * i] = amp * cs;
+        uh[2 * i + 1] = amp * sn;
     }
 }
 
@@ -150,6 +153,8 @@ int main(int argc, char** argv) {
         return 1;
     }
 
+    /* the Dormand-Prince scheme chooses dt itself: step by step, like the CFL-adaptive loop */
+    const int var_dt = r.adaptive || r.scheme == NSG_SCHEME_DP5;
     /* initial condition (InitialConditions, Solver/src/solver.c:1537-1648) or restart */
     double t0 = r.t0, dt = r.dt;
     int save_idx = 0;
@@ -159,6 +164,9 @@ int main(int argc, char** argv) {
             nsg_destroy(ctx);
             return 1;
         }
+        /* fixed-dt runs step with -h (the loop counts steps of r.dt); a variable-dt run continues
+         * from the step size the file was saved with */
+        if (!var_dt) dt = r.dt;
         CHECK(nsg_set_state(ctx, uh));
         CHECK(nsg_set_time(ctx, t0, dt));
         r.t0 = t0;
@@ -178,13 +186,25 @@ int main(int argc, char** argv) {
         CHECK(nsg_set_state(ctx, uh));
     }
 
-    /* integration counters (InitializeIntegrationVariables, Solver/src/solver.c:1314-1363) */
+    /* integration counters (InitializeIntegrationVariables, Solver/src/solver.c:1314-1363); with
+     * --transients the first TRANS_FRAC of the steps are not saved and the initial condition is
+     * not written (:1339-1348, Solver/src/hdf5_funcs.c:117) */
     const long num_t_steps = (long)((r.T - r.t0) / r.dt);
-    long num_print_steps = (num_t_steps >= r.save_every) ? num_t_steps / r.save_every + 1 : num_t_steps + 1;
-    /* the Dormand-Prince scheme chooses dt itself: step by step, like the CFL-adaptive loop */
-    const int var_dt = r.adaptive || r.scheme == NSG_SCHEME_DP5;
+    const long trans_steps = r.transients ? (long)(0.2 * (double)num_t_steps) : 0;
+    long num_print_steps;
+    if (r.transients)
+        num_print_steps = (num_t_steps >= r.save_every) ? (num_t_steps - trans_steps) / r.save_every
+                                                        : num_t_steps - trans_steps;
+    else
+        num_print_steps = (num_t_steps >= r.save_every) ? num_t_steps / r.save_every + 1 : num_t_steps + 1;
     if (var_dt) num_print_steps *= 2; /* room for extra saves, as __ADAPTIVE_STEP does (:1375-1379) */
-    if (!r.quiet) printf("Total Iters: %ld\t Saving Iters: %ld\n", num_t_steps, num_print_steps);
+    if (!r.quiet) {
+        if (r.transients)
+            printf("Total Iters: %ld\t Saving Iters: %ld\t Transient Steps: %ld\n", num_t_steps, num_print_steps,
+                   trans_steps);
+        else
+            printf("Total Iters: %ld\t Saving Iters: %ld\n", num_t_steps, num_print_steps);
+    }
     series s = {0};
     s.cap = num_print_steps;
     s.time = calloc((size_t)s.cap, sizeof(double)); s.E = calloc((size_t)s.cap, sizeof(double));
@@ -197,12 +217,15 @@ int main(int argc, char** argv) {
     if (nsg_h5_open(&h, &r, tag) != 0) { nsg_destroy(ctx); return 1; }
     if (!r.quiet) printf("\nMain Output File: %s\n\n", h.main_name);
 
-    int rc = record(ctx, &r, &h, &s, save_idx++, r.t0, dt, N, uh, wh, ur, wr, ek, ens, nspect, 0, num_t_steps, r.T);
+    int rc = NSG_OK;
+    if (!r.transients)
+        rc = record(ctx, &r, &h, &s, save_idx++, r.t0, dt, N, uh, wh, ur, wr, ek, ens, nspect, 0, num_t_steps, r.T);
     /* time loop */
     double t = r.t0 + dt;
     long iters = 1, pending = 0;
     while (rc == NSG_OK && t <= r.T) {
-        const int save = (iters % r.save_every) == 0;
+        const int save = iters > trans_steps && (iters % r.save_every) == 0;
+        if (r.transients && iters == trans_steps && !r.quiet) printf("\n\n...Transient Iterations Complete!\n\n");
         if (var_dt) {
             rc = nsg_step(ctx, 1);
             if (rc == NSG_OK) nsg_get_time(ctx, &t, &dt, NULL);

@@ -37,6 +37,8 @@ typedef struct nsg_run {
     int spectra;         /* write Spectra_HDF_Data.h5 (--no-spectra turns off) */
     int quirk;           /* --quirk-measures: reproduce the reference's measure arithmetic */
     double max_speed;    /* --max-speed blow-up guard */
+    int transients;      /* --transients: the reference's -D__TRANSIENTS build (Solver/src/data_types.h:76-77):
+                            the first TRANS_FRAC = 0.2 of the steps are not saved (Solver/src/solver.c:1339-1348) */
     int quiet;
 } nsg_run;
 

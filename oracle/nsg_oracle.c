@@ -7,6 +7,7 @@
  * the oracle's own recursive mixed-radix transform (no FFTW in this image); it is pinned against
  * numpy.fft (pocketfft) golden vectors and brute-force DFTs in tests/test_oracle_fft.py.
  */
+#define _GNU_SOURCE /* sincos */
 #include "nsg_oracle.h"
 
 #include <math.h>
@@ -843,9 +844,11 @@ void oracle_ic_shapiro(int N, double nu, double t, double* uhd) {
     free(u);
 }
 
-/* default RANDOM branch, Solver/src/solver.c:1551,1605-1624: srand(123456789); per mode and     */
-/* component amplitude U[0,1] then phase U[0,1] (draw order within the expression is             */
-/* unspecified C; amplitude-first is assumed: parity unpinned), then dealias.                    */
+/* default RANDOM branch, Solver/src/solver.c:1551,1605-1624: srand(123456789); per mode (i, j, */
+/* kz) and component d (innermost) amplitude U[0,1] then phase U[0,1], U = rand()/RAND_MAX, then */
+/* dealias.  The draw order inside the reference's expression is the one gcc evaluates (left     */
+/* factor first); cexp of a purely imaginary argument is glibc's sincos of it.  Both are pinned  */
+/* in tests/test_oracle_random_ic.py against the glibc stream and a gcc-compiled probe.          */
 void oracle_ic_random_ref(int N, double* uhd) {
     const int Nzf = N / 2 + 1;
     cplx* uh = (cplx*)uhd;
@@ -853,8 +856,10 @@ void oracle_ic_random_ref(int N, double* uhd) {
     for (long n = 0; n < (long)N * N * Nzf * 3; ++n) {
         double amp = (double)rand() / (double)RAND_MAX;
         double ph = (double)rand() / (double)RAND_MAX;
-        uh[n].re = amp * cos(ph * 2.0 * M_PI);
-        uh[n].im = amp * sin(ph * 2.0 * M_PI);
+        double sn, cs;
+        sincos(ph * 2.0 * M_PI, &sn, &cs);
+        uh[n].re = amp * cs;
+        uh[n].im = amp * sn;
     }
     oracle_dealias(N, uhd);
 }

@@ -24,7 +24,8 @@
  *
  * Parity pins: every function is checked by tests/test_oracle_*.py against numpy golden vectors
  * (tests/golden/), brute-force DFTs, closed forms (Taylor-Green, Beltrami/Shapiro) or invariants.
- * oracle_ic_random_ref is "parity unpinned" (the reference's rand() draw order is unspecified C).
+ * oracle_ic_random_ref is pinned to the glibc rand() stream and to the draw order gcc gives the
+ * reference's expression (tests/test_oracle_random_ic.py).
  * D18-D20 (Dormand-Prince tableau, error estimate and step control) are listed at oracle_step_rk5
  * and oracle_dp5_solve in nsg_oracle.c.
  */

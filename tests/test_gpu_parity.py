@@ -179,7 +179,7 @@ def resolved(ek, floor=1e-10):
     return ek >= floor * ek.max()
 
 
-@pytest.mark.parametrize("N", [32, 64, 128])
+@pytest.mark.parametrize("N", [8, 12, 16, 24, 32, 64, 128])
 def test_fp32bulk_errors(nsg, oracle, N):
     """FP32-bulk vs the FP64 oracle from the same state: rhs rel L2 <= 2e-6, one-step state rel
     L2 <= 1e-8, E(k) rel <= 1e-5 on every resolved shell, totals rel <= 1e-7."""

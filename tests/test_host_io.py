@@ -92,3 +92,37 @@ def test_file_only_naming(harness):
                          text=True)
     assert out.returncode == 0, out.stderr
     assert "Main_HDF_Data_SIM[NAV-RK4-FULL]_N[8,8]_T[0-1]_NU[1.0000000000]_CFL[1.73]_u0[TG_VORT].h5" in out.stdout
+
+
+CLI_PRINT = r"""
+#include <stdio.h>
+#include "nsg_run.h"
+int main(int argc, char** argv) {
+    nsg_run r;
+    int rc = nsg_parse_args(argc, argv, &r);
+    if (rc) return rc;
+    printf("%d %d %d %d %s\n", r.visc_pow, r.transients, r.scheme, r.save_every, r.u0);
+    return 0;
+}
+"""
+
+
+@pytest.fixture(scope="module")
+def cli_print(tmp_path_factory):
+    d = tmp_path_factory.mktemp("cli")
+    src, exe = str(d / "cli.c"), str(d / "cli")
+    with open(src, "w") as f:
+        f.write(CLI_PRINT)
+    subprocess.run(["gcc", "-O1", "-std=c11", "-D_DEFAULT_SOURCE", f"-I{HOST}", src, os.path.join(HOST, "nsg_cli.c"),
+                    "-o", exe, "-lm"], check=True, capture_output=True)
+    return exe
+
+
+def test_cli_defaults_follow_reference_build(cli_print):
+    """no flags: the reference's `make solver` physics (hyperviscous, SOLVER_FLAGS -D__HYPER,
+    Solver/Makefile:46), RK4, saves every 100 steps (Solver/src/utils.c:45-67), no transients"""
+    out = subprocess.run([cli_print], capture_output=True, text=True, check=True).stdout.split()
+    assert out == ["2", "0", "0", "100", "TG_VORT"]
+    out = subprocess.run([cli_print, "--visc-pow", "1", "--transients", "-i", "BOGUS"], capture_output=True, text=True,
+                         check=True).stdout.split()
+    assert out == ["1", "1", "0", "100", "NONE"]

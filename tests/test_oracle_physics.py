@@ -137,6 +137,52 @@ def test_ifrk4_fourth_order(oracle):
     assert 12.0 < e1 / e2 < 20.0
 
 
+def _rk4_planted(oracle, u, dt, n, a43):
+    """numpy RK4 over the oracle's nonlinear term with the last stage's input coefficient a43
+    (1 in RK4; a planted mistake otherwise), nu = 0"""
+    for _ in range(n):
+        k1 = oracle.rhs(u)
+        k2 = oracle.rhs(u + 0.5 * dt * k1)
+        k3 = oracle.rhs(u + 0.5 * dt * k2)
+        k4 = oracle.rhs(u + a43 * dt * k3)
+        u = u + dt * (k1 / 6 + k2 / 3 + k3 / 3 + k4 / 6)
+    return u
+
+
+def test_rk4cn_fourth_order_inviscid(oracle):
+    """RK4Step (Solver/src/solver.c:505-608) with nu = 0: D = 0 makes the Crank-Nicolson factor 1
+    and the scheme is the classical RK4 on the nonlinear term, so the self-convergence error ratio
+    for dt -> dt/2 is about 2^4.  The stage coefficients (1/2, 1/2, 1) and weights (1/6, 1/3, 1/3,
+    1/6) are all exercised: the same test on a copy with the last stage's input at 0.5 dt instead
+    of dt (a plausible slip) drops to second order, and the oracle equals the classical scheme run
+    over its own nonlinear term."""
+    N, T = 12, 0.4
+    u0 = oracle.ic_taylor_green(N) * 3.0
+    u0 = u0 + 0.3 * oracle.ic_k4(N, 2.0, 0.5, 21)  # break the TG symmetries
+    ref = oracle.step(u0, 0.0, T / 64, 1, "rk4cn", nsteps=64)
+    e1 = rel(oracle.step(u0, 0.0, T / 4, 1, "rk4cn", nsteps=4), ref)
+    e2 = rel(oracle.step(u0, 0.0, T / 8, 1, "rk4cn", nsteps=8), ref)
+    e3 = rel(oracle.step(u0, 0.0, T / 16, 1, "rk4cn", nsteps=16), ref)
+    assert 13.0 < e1 / e2 < 19.0 and 13.0 < e2 / e3 < 19.0
+    assert rel(oracle.step(u0, 0.0, T / 8, 1, "rk4cn", nsteps=8), _rk4_planted(oracle, u0, T / 8, 8, 1.0)) < 1e-13
+    ref_w = _rk4_planted(oracle, u0, T / 64, 64, 0.5)
+    w1 = rel(_rk4_planted(oracle, u0, T / 4, 4, 0.5), ref_w)
+    w2 = rel(_rk4_planted(oracle, u0, T / 8, 8, 0.5), ref_w)
+    assert w1 / w2 < 9.0
+
+
+def test_rk4cn_viscous_factor(oracle):
+    """single mode of a linear problem: a Beltrami field has NL = 0, so one RK4-CN step multiplies
+    the mode by (2 - D)/(2 + D) with D = dt nu |k|^(2p) (Solver/src/solver.c:590-603) -- for both
+    viscosity powers, which separates the factor from e^{-D} and from (1 - D/2)/(1 + D/2)^2"""
+    N, dt = 12, 0.05
+    u0 = oracle.ic_shapiro(N)
+    for p, nu in [(1, 0.3), (2, 0.02)]:
+        D = dt * nu * 12.0 ** p
+        got = oracle.step(u0, nu, dt, p, "rk4cn")
+        assert rel(got, u0 * (2 - D) / (2 + D)) < 1e-13
+
+
 # ---------------------------------------------------------------------------- invariants
 @pytest.mark.parametrize("N", [12, 16])
 def test_rhs_invariants(oracle, golden, N):
