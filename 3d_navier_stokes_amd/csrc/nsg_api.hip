@@ -248,9 +248,22 @@ template <typename T, int N, int S> static constexpr bool zrhs_persistent() {
     return TPFZ <= 64 && 64 % TPFZ == 0 && ((N / 2) & (N / 2 - 1)) == 0;
 }
 
+// Full-length paired z pass (passz_full_kernel): elements per thread of its N-point transforms, or
+// 0 where the half-length form is kept.  Measured at 512^3 (isolated launches, same inputs): FP64
+// 1.33 ms against 1.65 ms (the half-length form is bound by its 27 LDS exchanges per line); FP32
+// 0.745 ms against 0.84 ms.  At 256^3 both forms take the same time within 5 %; the full form
+// is used for both precisions there as well.  Other sizes keep the half-length form.
+template <typename T, int N> static constexpr int zfull_ept() {
+    if constexpr (is64<T>()) return N == 256 ? 4 : N == 512 ? 8 : 0;
+    else return (N == 256 || N == 512) ? 8 : 0;
+}
+// waves per SIMD the full form is compiled for (no spills at these bounds; FP64 at 512: 256 VGPRs)
+template <typename T, int N> static constexpr int zfull_occ() { return (is64<T>() && N == 512) ? 2 : 3; }
+
 template <typename T, int N, int S>
 static int launch_passz_rhs(nsg_ctx* c, int b0, int nb, double scale, int track_max) {
-    constexpr int EPTZ = EZof<T, N, S>(), TPFZ = N / 2 / EPTZ, LPW = 64 / TPFZ, WPB = 4;
+    constexpr int EF = zfull_ept<T, N>();
+    constexpr int EPTZ = EZof<T, N, S>(), TPFZ = EF ? N / EF : N / 2 / EPTZ, LPW = 64 / TPFZ, WPB = 4;
     static_assert(EPTZ >= 1 && LPW >= 1, "bad z configuration");
     static_assert(Ext<N>::NZP == nzp_of(N), "the kernels' compile-time extents must match the context's");
     PZArgs a;
@@ -263,6 +276,14 @@ static int launch_passz_rhs(nsg_ctx* c, int b0, int nb, double scale, int track_
     long long blocks = (groups + WPB - 1) / WPB;
     const long long cap = (long long)c->ncu * c->zblocks_per_cu;
     if (blocks > cap) blocks = cap;
+    if constexpr (EF > 0) {
+        prof_begin(c);
+        hipLaunchKernelGGL((passz_full_kernel<T, N, EF, WPB, zfull_occ<T, N>(), true, false>), dim3((unsigned)blocks),
+                           dim3(64 * WPB), 0, c->s, a);
+        LAUNCHCHK(c);
+        prof_end(c, 2);
+        return NSG_OK;
+    }
     prof_begin(c);
     // Fields of at most 16 VGPRs per thread: compiled for 2 waves per SIMD.  The kernel's natural
     // allocation sits at 250-256 VGPRs, and a build one register over would fall to 1 wave per

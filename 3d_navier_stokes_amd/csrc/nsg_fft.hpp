@@ -308,4 +308,90 @@ __device__ __forceinline__ void fft_reg(C (&v)[EPT], int t, C* lds, int base, in
     fft_stages_reg<L, EPT, TPF, 0, num_stages(L, EPT), DIR, SYNC, PAD, NTW>(v, t, lds, base, PS, w);
 }
 
+// ------------------------------------------------------------------------------------------
+// Full-table variant: every twiddle W_{PR}^{q (b mod P)} (q = 1..R-1) of every butterfly of
+// stages >= 1 is held in registers (fft_load_twiddle_table, once per kernel), so a transform issues
+// no twiddle arithmetic besides the products themselves.  Costs sum_s (R_s - 1) EPT / R_s complex
+// registers per thread.
+// ------------------------------------------------------------------------------------------
+constexpr int num_table_twiddles(int L, int ept) {
+    int n = 0;
+    for (int s = 1; s < num_stages(L, ept); ++s) n += (stage_radix(L, ept, s) - 1) * (ept / stage_radix(L, ept, s));
+    return n;
+}
+constexpr int ntab(int L, int ept) { return num_table_twiddles(L, ept) > 0 ? num_table_twiddles(L, ept) : 1; }
+template <int L, int EPT, int S> constexpr int table_twiddle_offset() {
+    int n = 0;
+    for (int s = 1; s < S; ++s) n += (stage_radix(L, EPT, s) - 1) * (EPT / stage_radix(L, EPT, s));
+    return n;
+}
+
+template <int L, int EPT, int TPF, int S, typename C>
+__device__ __forceinline__ void load_table_stage(C (&w)[ntab(L, EPT)], int t, const C* __restrict__ tw) {
+    if constexpr (S < num_stages(L, EPT)) {
+        constexpr int R = stage_radix(L, EPT, S);
+        constexpr int P = stage_p(L, EPT, S);
+        constexpr int B = EPT / R;
+        constexpr int off = table_twiddle_offset<L, EPT, S>();
+#pragma unroll
+        for (int beta = 0; beta < B; ++beta) {
+            const int k = (t + TPF * beta) % P;
+#pragma unroll
+            for (int q = 1; q < R; ++q) w[off + beta * (R - 1) + q - 1] = tw[q * k * (L / (P * R))];
+        }
+        load_table_stage<L, EPT, TPF, S + 1>(w, t, tw);
+    }
+}
+// w[] receives exp(-2 pi i q (b mod P)/(P R)) for q = 1..R-1 and every butterfly b of stages
+// 1..NS-1 (forward sign; the inverse conjugates on use); tw is the length-L table exp(-2 pi i e/L)
+template <int L, int EPT, typename C>
+__device__ __forceinline__ void fft_load_twiddle_table(C (&w)[ntab(L, EPT)], int t, const C* __restrict__ tw) {
+    load_table_stage<L, EPT, L / EPT, 1>(w, t, tw);
+}
+
+template <int L, int EPT, int S, int DIR, typename C>
+__device__ __forceinline__ void stage_compute_tab(C (&v)[EPT], const C (&w)[ntab(L, EPT)]) {
+    constexpr int R = stage_radix(L, EPT, S);
+    constexpr int B = EPT / R;
+#pragma unroll
+    for (int beta = 0; beta < B; ++beta) {
+        if constexpr (S > 0) {
+            constexpr int off = table_twiddle_offset<L, EPT, S>();
+#pragma unroll
+            for (int q = 1; q < R; ++q) {
+                C wq = w[off + beta * (R - 1) + q - 1];
+                if constexpr (DIR > 0) wq.y = -wq.y;
+                v[beta + B * q] = cmul(v[beta + B * q], wq);
+            }
+        }
+        if constexpr (R == 2) dft2<DIR>(v[beta], v[beta + B]);
+        else if constexpr (R == 3) dft3<DIR>(v[beta], v[beta + B], v[beta + 2 * B]);
+        else if constexpr (R == 4) dft4<DIR>(v[beta], v[beta + B], v[beta + 2 * B], v[beta + 3 * B]);
+        else if constexpr (R == 8)
+            dft8<DIR>(v[beta], v[beta + B], v[beta + 2 * B], v[beta + 3 * B], v[beta + 4 * B], v[beta + 5 * B],
+                      v[beta + 6 * B], v[beta + 7 * B]);
+    }
+}
+
+template <int L, int EPT, int TPF, int S, int NS, int DIR, int SYNC, bool PAD, typename C>
+__device__ __forceinline__ void fft_stages_tab(C (&v)[EPT], int t, C* lds, int base, int PS, const C (&w)[ntab(L, EPT)]) {
+    if constexpr (S < NS) {
+        stage_compute_tab<L, EPT, S, DIR>(v, w);
+        if constexpr (S + 1 < NS) {
+            stage_scatter<L, EPT, TPF, S, PAD>(v, t, lds, base, PS);
+            lds_sync<SYNC>();
+            gather_pattern<L, EPT, TPF, S, PAD>(v, t, lds, base, PS);
+            lds_sync<SYNC>();
+        }
+        fft_stages_tab<L, EPT, TPF, S + 1, NS, DIR, SYNC, PAD>(v, t, lds, base, PS, w);
+    }
+}
+
+template <int L, int EPT, int DIR, int SYNC, bool PAD, typename C>
+__device__ __forceinline__ void fft_tab(C (&v)[EPT], int t, C* lds, int base, int PS, const C (&w)[ntab(L, EPT)]) {
+    static_assert(schedule_ok(L, EPT), "unsupported FFT length / elements-per-thread");
+    constexpr int TPF = L / EPT;
+    fft_stages_tab<L, EPT, TPF, 0, num_stages(L, EPT), DIR, SYNC, PAD>(v, t, lds, base, PS, w);
+}
+
 }  // namespace nsg

@@ -539,10 +539,19 @@ __device__ __forceinline__ void zr_load(cpx_t<T> (&x)[EPTZ], const cpx_t<T>* lin
 
 // c2r of one field whose X[k] (k = t + TPF m) are in x; result z[n] = x[2n] + i x[2n+1] in v
 template <int M, int EPTZ> constexpr int zntw() { return num_base_twiddles(M, EPTZ) > 0 ? num_base_twiddles(M, EPTZ) : 1; }
+// twiddle registers of the half-length transform: base twiddles (powers formed by multiplication)
+// or, with TAB, the full table of every butterfly's twiddles
+template <int M, int EPTZ, bool TAB> constexpr int zntw2() { return TAB ? ntab(M, EPTZ) : zntw<M, EPTZ>(); }
 
-template <typename T, int N, int EPTZ, int SY>
+template <int M, int EPTZ, int DIR, int SY, bool TAB, typename C>
+__device__ __forceinline__ void zfft(C (&v)[EPTZ], int t, C* lds, int base, const C (&w)[zntw2<M, EPTZ, TAB>()]) {
+    if constexpr (TAB) fft_tab<M, EPTZ, DIR, SY, true>(v, t, lds, base, 1, w);
+    else fft_reg<M, EPTZ, DIR, SY, true, zntw<M, EPTZ>()>(v, t, lds, base, 1, w);
+}
+
+template <typename T, int N, int EPTZ, int SY, bool TAB = false>
 __device__ __forceinline__ void zr_c2r(cpx_t<T> (&v)[EPTZ], const cpx_t<T> (&x)[EPTZ], int t, cpx_t<T>* lds, int base,
-                                       const cpx_t<T> (&wz)[zntw<N / 2, EPTZ>()], const cpx_t<T> (&ek)[EPTZ]) {
+                                       const cpx_t<T> (&wz)[zntw2<N / 2, EPTZ, TAB>()], const cpx_t<T> (&ek)[EPTZ]) {
     using C = cpx_t<T>;
     constexpr int M = N / 2, TPF = M / EPTZ;
 #pragma unroll
@@ -562,17 +571,17 @@ __device__ __forceinline__ void zr_c2r(cpx_t<T> (&v)[EPTZ], const cpx_t<T> (&x)[
         v[m] = cmk<C>(sx + e.y * dx - e.x * dy, sy + e.y * dy + e.x * dx);
     }
     lds_sync<SY>();
-    fft_reg<M, EPTZ, +1, SY, true>(v, t, lds, base, 1, wz);
+    zfft<M, EPTZ, +1, SY, TAB>(v, t, lds, base, wz);
 }
 
-template <typename T, int N, int EPTZ, int SY>
+template <typename T, int N, int EPTZ, int SY, bool TAB = false>
 __device__ __forceinline__ void zr_r2c_store(cpx_t<T> (&v)[EPTZ], cpx_t<T>* outline, int kmax, int NZP, int t,
-                                             cpx_t<T>* lds, int base, const cpx_t<T> (&wz)[zntw<N / 2, EPTZ>()],
+                                             cpx_t<T>* lds, int base, const cpx_t<T> (&wz)[zntw2<N / 2, EPTZ, TAB>()],
                                              const cpx_t<T> (&ek)[EPTZ], bool live) {
     using C = cpx_t<T>;
     using R = decltype(C::x);
     constexpr int M = N / 2, TPF = M / EPTZ;
-    fft_reg<M, EPTZ, -1, SY, true>(v, t, lds, base, 1, wz);
+    zfft<M, EPTZ, -1, SY, TAB>(v, t, lds, base, wz);
 #pragma unroll
     for (int m = 0; m < EPTZ; ++m) lds[base + t + TPF * m] = v[m];
     lds_sync<SY>();
@@ -700,6 +709,328 @@ __global__ void __launch_bounds__(64 * WPB, OCC) passz_rhs_kernel(PZArgs a) {
         C* out = out0 + line_off(gl, a.obox);
 #pragma unroll
         for (int c = 0; c < 3; ++c) zr_r2c_store<T, N, EPTZ, SY>(pr[c], out + c * fstride, kmax, NZP, t, lds, base, wz, ek, live);
+    }
+    flush_max();
+}
+
+// ------------------------------------------------------------------------------------------
+// Pass 3, RHS form, reduced-liveness persistent version.  Same lines, exchanges and outputs as
+// passz_rhs_kernel, but the six fields are transformed in the order u_y, w_z, u_z, w_y, u_x, w_x
+// so that P_x = u_y w_z - u_z w_y is complete after four fields and is transformed back and stored
+// before u_x arrives; then P_y = u_z w_x - u_x w_z and P_z = u_x w_y - u_y w_x.  At most six
+// fields are live (three u/w fields, P_x, the transform in progress and the prefetched next field)
+// instead of nine, which leaves room for the full twiddle table (TAB) and for more waves per SIMD.
+// ------------------------------------------------------------------------------------------
+template <typename T, int N, int EPTZ, int WPB, int OCC, bool TAB>
+__global__ void __launch_bounds__(64 * WPB, OCC) passz_rhs2_kernel(PZArgs a) {
+    using C = cpx_t<T>;
+    constexpr int M = N / 2, TPF = M / EPTZ, LPW = 64 / TPF;
+    static_assert(TPF <= 64 && 64 % TPF == 0 && (M & (M - 1)) == 0, "line must fit a wavefront, M power of two");
+    constexpr int SY = SYNC_WAVE;
+    constexpr int MP = zlds<M, EPTZ>();
+    __shared__ C lds[WPB * LPW * MP];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int sub = lane / TPF, t = lane % TPF;
+    C wz[zntw2<M, EPTZ, TAB>()];
+    if constexpr (TAB) fft_load_twiddle_table<M, EPTZ>(wz, t, (const C*)a.twM);
+    else fft_load_base_twiddles<M, EPTZ>(wz, t, (const C*)a.twM);
+    C ek[EPTZ];  // e^{-2 pi i k / N}, k = t + TPF m
+#pragma unroll
+    for (int m = 0; m < EPTZ; ++m) ek[m] = ((const C*)a.twN)[t + TPF * m];
+    const int base = (wv * LPW + sub) * MP;
+    constexpr int kmax = Ext<N>::kmax, NZP = Ext<N>::NZP;
+    const long long nlines = (long long)N * N;  // lines per box (a multiple of LPW)
+    const long long total = nlines * a.nbox;
+    const long long ngroups = (total + LPW - 1) / LPW;
+    const long long fstride = nlines * NZP;
+    const C* in0 = (const C*)a.in;
+    C* out0 = (C*)a.out;
+    auto line_off = [&](long long gl, long long stride) {
+        const long long b = gl / nlines;
+        return b * stride + (gl - b * nlines) * NZP;
+    };
+    double mx = 0.0;
+    int mx_box = -1;
+    auto flush_max = [&]() {
+        if (a.track_max && mx_box >= 0) {
+            double v = mx * a.scale;
+            for (int off = 32; off > 0; off >>= 1) v = nanmax(v, __shfl_xor(v, off));
+            if (lane == 0) atomicMax(&a.sc[mx_box].maxbits, (unsigned long long)__double_as_longlong(v));
+        }
+        mx = 0.0;
+    };
+
+    long long g = (long long)blockIdx.x * WPB + wv;
+    C cur[EPTZ];  // coefficients of the field transformed next
+    if (g < ngroups) {
+        long long gl = g * LPW + sub;
+        if (gl >= total) gl = total - 1;
+        zr_load<T, N, EPTZ>(cur, in0 + line_off(gl, a.ibox) + 1 * fstride, kmax, t);  // u_y
+    }
+    for (; g < ngroups; g += (long long)gridDim.x * WPB) {
+        long long gl = g * LPW + sub;
+        const bool live = gl < total;
+        if (!live) gl = total - 1;  // idle lanes recompute the last line but never store
+        const int box = (int)((g * LPW) / nlines);
+        if (box != mx_box) {
+            flush_max();
+            mx_box = box;
+        }
+        const C* in = in0 + line_off(gl, a.ibox);
+        C* out = out0 + line_off(gl, a.obox);
+        C nxt[EPTZ], v[EPTZ], uy[EPTZ], wzf[EPTZ], uz[EPTZ], wy[EPTZ], px[EPTZ], py[EPTZ], pz[EPTZ];
+        // u_y
+        zr_load<T, N, EPTZ>(nxt, in + 5 * fstride, kmax, t);
+        zr_c2r<T, N, EPTZ, SY, TAB>(uy, cur, t, lds, base, wz, ek);
+        // w_z: P_x = u_y w_z
+        zr_load<T, N, EPTZ>(cur, in + 2 * fstride, kmax, t);
+        zr_c2r<T, N, EPTZ, SY, TAB>(wzf, nxt, t, lds, base, wz, ek);
+#pragma unroll
+        for (int m = 0; m < EPTZ; ++m) px[m] = cmk<C>(uy[m].x * wzf[m].x, uy[m].y * wzf[m].y);
+        // u_z
+        zr_load<T, N, EPTZ>(nxt, in + 4 * fstride, kmax, t);
+        zr_c2r<T, N, EPTZ, SY, TAB>(uz, cur, t, lds, base, wz, ek);
+        // w_y: P_x -= u_z w_y, complete
+        zr_load<T, N, EPTZ>(cur, in + 0 * fstride, kmax, t);
+        zr_c2r<T, N, EPTZ, SY, TAB>(wy, nxt, t, lds, base, wz, ek);
+#pragma unroll
+        for (int m = 0; m < EPTZ; ++m) px[m] = cmk<C>(px[m].x - uz[m].x * wy[m].x, px[m].y - uz[m].y * wy[m].y);
+        zr_r2c_store<T, N, EPTZ, SY, TAB>(px, out + 0 * fstride, kmax, NZP, t, lds, base, wz, ek, live);
+        // u_x: P_y = -u_x w_z, P_z = u_x w_y; the running max of |u_x| + |u_y| + |u_z|
+        zr_load<T, N, EPTZ>(nxt, in + 3 * fstride, kmax, t);
+        zr_c2r<T, N, EPTZ, SY, TAB>(v, cur, t, lds, base, wz, ek);
+#pragma unroll
+        for (int m = 0; m < EPTZ; ++m) {
+            py[m] = cmk<C>(-(v[m].x * wzf[m].x), -(v[m].y * wzf[m].y));
+            pz[m] = cmk<C>(v[m].x * wy[m].x, v[m].y * wy[m].y);
+        }
+        if (a.track_max && live) {
+#pragma unroll
+            for (int m = 0; m < EPTZ; ++m) {
+                const double s0 = fabs((double)v[m].x) + fabs((double)uy[m].x) + fabs((double)uz[m].x);
+                const double s1 = fabs((double)v[m].y) + fabs((double)uy[m].y) + fabs((double)uz[m].y);
+                mx = nanmax(mx, nanmax(s0, s1));
+            }
+        }
+        // w_x: P_y += u_z w_x, P_z -= u_y w_x; meanwhile u_y of the next line group is fetched
+        const long long gn = g + (long long)gridDim.x * WPB;
+        if (gn < ngroups) {
+            long long ln = gn * LPW + sub;
+            if (ln >= total) ln = total - 1;
+            zr_load<T, N, EPTZ>(cur, in0 + line_off(ln, a.ibox) + 1 * fstride, kmax, t);
+        }
+        zr_c2r<T, N, EPTZ, SY, TAB>(v, nxt, t, lds, base, wz, ek);
+#pragma unroll
+        for (int m = 0; m < EPTZ; ++m) {
+            py[m] = cmk<C>(py[m].x + uz[m].x * v[m].x, py[m].y + uz[m].y * v[m].y);
+            pz[m] = cmk<C>(pz[m].x - uy[m].x * v[m].x, pz[m].y - uy[m].y * v[m].y);
+        }
+        zr_r2c_store<T, N, EPTZ, SY, TAB>(py, out + 1 * fstride, kmax, NZP, t, lds, base, wz, ek, live);
+        zr_r2c_store<T, N, EPTZ, SY, TAB>(pz, out + 2 * fstride, kmax, NZP, t, lds, base, wz, ek, live);
+    }
+    flush_max();
+}
+
+// ------------------------------------------------------------------------------------------
+// Pass 3, RHS form, full-length paired version.  Two real fields share one complex transform of
+// length N: the inverse of Z = U_h + i W_h (U_h, W_h the Hermitian extensions of two half spectra,
+// U_h[N - k] = conj U[k]) is u + i w, and the forward transform Z of p + i q gives
+// P[k] = (Z[k] + conj Z[N - k]) / 2, Q[k] = (Z[k] - conj Z[N - k]) / (2 i).  Per line: three
+// inverse transforms (u_y, w_z), (u_z, w_y), (u_x, w_x) -- each pair's product terms are formed as
+// soon as it arrives -- and two forward transforms, (P_x, P_y) and P_z alone.  Against the
+// half-length form this removes the pre/post rotation of every field and its LDS exchange: 5
+// transforms of N points and 10 exchanges per line instead of 9 of N/2 points and 27 (radix 8 at
+// N = 512).  Only kz <= kmax is nonzero on input, so positions kmax < p < N - kmax are zero; the
+// element slots that are zero for every thread are compile-time zeros, and outputs beyond kmax
+// are never formed.  The mirrored coefficients X[N - p] are read from global memory (the line was
+// just fetched, so they come from cache); the separation partner Z[N - k] comes from the thread
+// holding position N - k by a cross-lane shuffle (thread TPF - t, element EPT - 1 - m).
+// One line per TPF = N / EPT threads; lines per wavefront LPW = 64 / TPF; persistent loop.
+// ------------------------------------------------------------------------------------------
+// raw coefficients of one field for the positions p = t + TPF m of a full-length transform:
+// X[p] for p <= kmax, X[N - p] for p >= N - kmax, zero between (compile-time zero where the whole
+// element slot lies between)
+template <typename T, int N, int EPT>
+__device__ __forceinline__ void zf_load(cpx_t<T> (&x)[EPT], const cpx_t<T>* line, int t) {
+    using C = cpx_t<T>;
+    constexpr int TPF = N / EPT, kmax = Ext<N>::kmax;
+#pragma unroll
+    for (int m = 0; m < EPT; ++m) {
+        if (TPF * m > kmax && TPF * m + TPF - 1 < N - kmax) {  // folded once the loop is unrolled
+            x[m] = cmk<C>(0, 0);
+        } else {
+            const int p = t + TPF * m;
+            const int src = (p <= kmax) ? p : ((p >= N - kmax) ? N - p : -1);
+            x[m] = (src >= 0) ? line[src] : cmk<C>(0, 0);
+        }
+    }
+}
+// Z[p] = U_h[p] + i W_h[p] from the raw coefficients (conjugated on the mirrored side; the
+// imaginary part of X[0] is dropped, as c2r does)
+template <typename T, int N, int EPT>
+__device__ __forceinline__ void zf_pack(cpx_t<T> (&z)[EPT], const cpx_t<T> (&u)[EPT], const cpx_t<T> (&w)[EPT], int t) {
+    using C = cpx_t<T>;
+    constexpr int TPF = N / EPT, kmax = Ext<N>::kmax;
+#pragma unroll
+    for (int m = 0; m < EPT; ++m) {
+        if (TPF * m > kmax && TPF * m + TPF - 1 < N - kmax) {  // folded once the loop is unrolled
+            z[m] = cmk<C>(0, 0);
+        } else {
+            const int p = t + TPF * m;
+            C a = u[m], b = w[m];
+            if (p > N / 2) { a.y = -a.y; b.y = -b.y; }
+            if (p == 0) { a.y = 0; b.y = 0; }
+            z[m] = cmk<C>(a.x - b.y, a.y + b.x);
+        }
+    }
+}
+template <int N, int EPT> constexpr int zf_mout() { return Ext<N>::kmax / (N / EPT) + 1; }  // slots reaching k <= kmax
+
+// forward transform of p + i q (pair) or of p alone (!PAIR), then the stores of kz < NZP
+template <typename T, int N, int EPT, int SY, bool TAB, bool PAIR>
+__device__ __forceinline__ void zf_forward_store(cpx_t<T> (&z)[EPT], cpx_t<T>* out1, cpx_t<T>* out2, int t, int lane,
+                                                 cpx_t<T>* lds, int base, const cpx_t<T> (&wz)[zntw2<N, EPT, TAB>()],
+                                                 bool live) {
+    using C = cpx_t<T>;
+    using R = decltype(C::x);
+    constexpr int TPF = N / EPT, kmax = Ext<N>::kmax, NZP = Ext<N>::NZP;
+    constexpr int MOUT = zf_mout<N, EPT>();
+    constexpr int MST = ((NZP < N ? NZP : N) - 1) / TPF + 1;  // slots holding some k < min(NZP, N)
+    zfft<N, EPT, -1, SY, TAB>(z, t, lds, base, wz);
+    C o1[MST], o2[MST];
+    if constexpr (PAIR) {
+        const int src = (lane - t) + ((TPF - t) & (TPF - 1));
+#pragma unroll
+        for (int m = 0; m < MOUT; ++m) {
+            C zp;  // Z[N - k], k = t + TPF m
+            zp.x = __shfl(z[EPT - 1 - m].x, src);
+            zp.y = __shfl(z[EPT - 1 - m].y, src);
+            if (t == 0) zp = z[(EPT - m) & (EPT - 1)];
+            const C zk = z[m];
+            o1[m] = cmk<C>(R(0.5) * (zk.x + zp.x), R(0.5) * (zk.y - zp.y));
+            o2[m] = cmk<C>(R(0.5) * (zk.y + zp.y), R(0.5) * (zp.x - zk.x));
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < MOUT; ++m) o1[m] = z[m];
+    }
+    if (!live) return;
+#pragma unroll
+    for (int m = 0; m < MST; ++m) {
+        const int k = t + TPF * m;
+        if (k < NZP) {
+            const bool in = (m < MOUT) && k <= kmax;
+            out1[k] = in ? o1[m < MOUT ? m : 0] : cmk<C>(0, 0);
+            if constexpr (PAIR) out2[k] = in ? o2[m < MOUT ? m : 0] : cmk<C>(0, 0);
+        }
+    }
+    if constexpr (NZP > N) {
+        for (int k = N + t; k < NZP; k += TPF) {
+            out1[k] = cmk<C>(0, 0);
+            if constexpr (PAIR) out2[k] = cmk<C>(0, 0);
+        }
+    }
+}
+
+// PF: the first pair of the next line group is fetched during the current group's third inverse
+// transform (otherwise at the top of each iteration, which frees its registers)
+template <typename T, int N, int EPT, int WPB, int OCC, bool TAB, bool PF = true>
+__global__ void __launch_bounds__(64 * WPB, OCC) passz_full_kernel(PZArgs a) {
+    using C = cpx_t<T>;
+    constexpr int TPF = N / EPT, LPW = 64 / TPF;
+    static_assert(TPF <= 64 && 64 % TPF == 0 && (N & (N - 1)) == 0, "line must fit a wavefront, N power of two");
+    constexpr int SY = SYNC_WAVE;
+    constexpr int MP = fft_lds_len(N, EPT, true);
+    __shared__ C lds[WPB * LPW * MP];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int sub = lane / TPF, t = lane % TPF;
+    C wz[zntw2<N, EPT, TAB>()];
+    if constexpr (TAB) fft_load_twiddle_table<N, EPT>(wz, t, (const C*)a.twN);
+    else fft_load_base_twiddles<N, EPT>(wz, t, (const C*)a.twN);
+    const int base = (wv * LPW + sub) * MP;
+    constexpr int NZP = Ext<N>::NZP;
+    const long long nlines = (long long)N * N;
+    const long long total = nlines * a.nbox;
+    const long long ngroups = (total + LPW - 1) / LPW;
+    const long long fstride = nlines * NZP;
+    const C* in0 = (const C*)a.in;
+    C* out0 = (C*)a.out;
+    auto line_off = [&](long long gl, long long stride) {
+        const long long b = gl / nlines;
+        return b * stride + (gl - b * nlines) * NZP;
+    };
+    double mx = 0.0;
+    int mx_box = -1;
+    auto flush_max = [&]() {
+        if (a.track_max && mx_box >= 0) {
+            double v = mx * a.scale;
+            for (int off = 32; off > 0; off >>= 1) v = nanmax(v, __shfl_xor(v, off));
+            if (lane == 0) atomicMax(&a.sc[mx_box].maxbits, (unsigned long long)__double_as_longlong(v));
+        }
+        mx = 0.0;
+    };
+
+    long long g = (long long)blockIdx.x * WPB + wv;
+    C cu[EPT], cw[EPT];  // raw coefficients of the next pair
+    if (PF && g < ngroups) {
+        long long gl = g * LPW + sub;
+        if (gl >= total) gl = total - 1;
+        const C* in = in0 + line_off(gl, a.ibox);
+        zf_load<T, N, EPT>(cu, in + 1 * fstride, t);  // u_y
+        zf_load<T, N, EPT>(cw, in + 5 * fstride, t);  // w_z
+    }
+    for (; g < ngroups; g += (long long)gridDim.x * WPB) {
+        long long gl = g * LPW + sub;
+        const bool live = gl < total;
+        if (!live) gl = total - 1;  // idle lanes recompute the last line but never store
+        const int box = (int)((g * LPW) / nlines);
+        if (box != mx_box) {
+            flush_max();
+            mx_box = box;
+        }
+        const C* in = in0 + line_off(gl, a.ibox);
+        C* out = out0 + line_off(gl, a.obox);
+        if constexpr (!PF) {
+            zf_load<T, N, EPT>(cu, in + 1 * fstride, t);
+            zf_load<T, N, EPT>(cw, in + 5 * fstride, t);
+        }
+        C a1[EPT], a2[EPT], a3[EPT], nu[EPT], nw[EPT];
+        // (u_y, w_z): P_x = u_y w_z
+        zf_load<T, N, EPT>(nu, in + 2 * fstride, t);  // u_z
+        zf_load<T, N, EPT>(nw, in + 4 * fstride, t);  // w_y
+        zf_pack<T, N, EPT>(a1, cu, cw, t);
+        zfft<N, EPT, +1, SY, TAB>(a1, t, lds, base, wz);
+        // (u_z, w_y): P_x -= u_z w_y
+        zf_load<T, N, EPT>(cu, in + 0 * fstride, t);  // u_x
+        zf_load<T, N, EPT>(cw, in + 3 * fstride, t);  // w_x
+        zf_pack<T, N, EPT>(a2, nu, nw, t);
+        zfft<N, EPT, +1, SY, TAB>(a2, t, lds, base, wz);
+        // (u_x, w_x); meanwhile (u_y, w_z) of the next line group is fetched
+        const long long gn = g + (long long)gridDim.x * WPB;
+        zf_pack<T, N, EPT>(a3, cu, cw, t);
+        if (PF && gn < ngroups) {
+            long long ln = gn * LPW + sub;
+            if (ln >= total) ln = total - 1;
+            const C* inn = in0 + line_off(ln, a.ibox);
+            zf_load<T, N, EPT>(cu, inn + 1 * fstride, t);
+            zf_load<T, N, EPT>(cw, inn + 5 * fstride, t);
+        }
+        zfft<N, EPT, +1, SY, TAB>(a3, t, lds, base, wz);
+        // a1 = (u_y, w_z), a2 = (u_z, w_y), a3 = (u_x, w_x) at the grid points t + TPF m
+        C pq[EPT], pz[EPT];
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) {
+            const auto uy = a1[m].x, wzv = a1[m].y, uz = a2[m].x, wy = a2[m].y, ux = a3[m].x, wx = a3[m].y;
+            pq[m] = cmk<C>(uy * wzv - uz * wy, uz * wx - ux * wzv);  // P_x + i P_y
+            pz[m] = cmk<C>(ux * wy - uy * wx, 0);
+        }
+        if (a.track_max && live) {
+#pragma unroll
+            for (int m = 0; m < EPT; ++m)
+                mx = nanmax(mx, fabs((double)a3[m].x) + fabs((double)a1[m].x) + fabs((double)a2[m].x));
+        }
+        zf_forward_store<T, N, EPT, SY, TAB, true>(pq, out + 0 * fstride, out + 1 * fstride, t, lane, lds, base, wz, live);
+        zf_forward_store<T, N, EPT, SY, TAB, false>(pz, out + 2 * fstride, nullptr, t, lane, lds, base, wz, live);
     }
     flush_max();
 }

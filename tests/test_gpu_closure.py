@@ -47,7 +47,7 @@ def test_param_count(nsg):
         assert np.array_equal(net.get_params(), q)
 
 
-@pytest.mark.parametrize("N,B", [(16, 1), (16, 2), (32, 1)])
+@pytest.mark.parametrize("N,B", [(16, 1), (16, 2), (32, 1), (64, 1)])
 def test_forward_vs_oracle(nsg, N, B):
     x, _ = rand_inputs(10 + N + B, B, N)
     with nsg.Network(N, max_batch=B, seed=5) as net:
@@ -59,10 +59,11 @@ def test_forward_vs_oracle(nsg, N, B):
     assert rel(got, exact) < 3e-2
 
 
-def test_loss_and_gradients_vs_oracle(nsg):
-    B = 2
-    x, t = rand_inputs(21, B)
-    with nsg.Network(NG, max_batch=B, seed=9) as net:
+@pytest.mark.parametrize("N,B", [(NG, 2), (64, 1)])
+def test_loss_and_gradients_vs_oracle(nsg, N, B):
+    """N = 64 is the LES grid of the closure configuration"""
+    x, t = rand_inputs(21, B, N)
+    with nsg.Network(N, max_batch=B, seed=9) as net:
         loss, g = net.loss_grad(x, t)
         params = oracle_params(net)
     lo, go = C.loss_and_grads(params, x.astype(np.float64), t.astype(np.float64), emulate_bf16=True)
@@ -95,7 +96,7 @@ def test_training_reduces_loss(nsg):
     assert l1 < l0 / 2, (l0, l1)
 
 
-@pytest.mark.parametrize("Nd,Nl,kc", [(48, 24, 8.0), (64, 32, 10.0)])
+@pytest.mark.parametrize("Nd,Nl,kc", [(48, 24, 8.0), (64, 32, 10.0), (128, 64, 21.0)])
 def test_sgs_sample_vs_oracle(nsg, oracle, Nd, Nl, kc):
     uh = oracle.ic_k4(Nd, 3.0, 0.5, 17)
     with nsg.Solver(Nd) as dns, nsg.Solver(Nl) as les:
@@ -108,8 +109,8 @@ def test_sgs_sample_vs_oracle(nsg, oracle, Nd, Nl, kc):
     assert rel(tau, tauo) < 1e-6
 
 
-def test_closure_term_vs_oracle(nsg, oracle):
-    N = 32
+@pytest.mark.parametrize("N", [32, 64])
+def test_closure_term_vs_oracle(nsg, oracle, N):
     uh = oracle.ic_k4(N, 3.0, 0.5, 23)
     sig_in, sig_out = 2.5, [0.05, 0.05, 0.05, 0.03, 0.03, 0.03]
     with nsg.Network(N, seed=8) as net, nsg.Solver(N, nu=1e-3, dt=1e-3) as les:
@@ -125,10 +126,11 @@ def test_closure_term_vs_oracle(nsg, oracle):
     assert rel(M, want) < 1e-2
 
 
-def test_les_step_with_closure(nsg, oracle):
+@pytest.mark.parametrize("N", [32, 64])
+def test_les_step_with_closure(nsg, oracle, N):
     """zero network: the step is bitwise the plain step; trained-free random network: the state
     change differs from the plain step's by dt M to first order"""
-    N, dt = 32, 1e-3
+    dt = 1e-3
     uh = oracle.ic_k4(N, 3.0, 0.5, 29)
     with nsg.Solver(N, nu=2e-3, dt=dt) as plain:
         plain.set_state(uh)
@@ -176,3 +178,32 @@ def test_trained_closure_beats_no_model(nsg):
         errs[name] = L.spectral_error(r["E_k"], ek_ref, kc)[1]
     net.close()
     assert errs["cnn"] < 0.5 * errs["none"], errs
+
+
+def test_les64_closure_run(nsg, oracle):
+    """the closure configuration's LES on its own grid: 64^3, forced (eps_in = 0.1 on |k| <= 2.5),
+    nu = 1e-3, CFL 0.5, with a network attached for 200 steps.  Every step evaluates the model
+    (three c2r of the gradients, four BF16 conv layers, two r2c); the run stays finite, injects
+    energy at the forcing rate to first order, and detaching the network restores the plain step
+    bitwise."""
+    N = 64
+    uh = oracle.ic_k4(N, 3.0, 0.5, 41)
+    with nsg.Network(N, seed=12) as net, nsg.Solver(N, nu=1e-3, dt=1e-3, cfl=0.5, force_eps=0.1) as les:
+        net.set_norm(2.5, [0.05] * 6)
+        net.set_params(np.random.default_rng(6).standard_normal(net.param_count).astype(np.float32) * 0.01)
+        les.set_state(uh)
+        les.attach_closure(net)
+        e0 = les.measures()[0].energy
+        les.step(200)
+        t, dt, mx = les.time()
+        tot = les.measures()[0]
+        assert np.isfinite(tot.energy) and np.isfinite(mx) and t > 0
+        assert 0.2 * e0 < tot.energy < 5.0 * e0
+        les.attach_closure(None)
+        u = les.get_state()
+        les.step(1)
+        u_les = les.get_state()
+    with nsg.Solver(N, nu=1e-3, dt=1e-3, cfl=0.5, force_eps=0.1) as plain:
+        plain.set_state(u)
+        plain.step(1)
+        assert np.array_equal(plain.get_state(), u_les)

@@ -1,6 +1,8 @@
-"""The persistent z pass (passz_rhs_kernel, the fused c2r -> u x w -> r2c of
-NonlinearRHSBatch, Solver/src/solver.c:656-683) run on the CPU through tests/zpass_emu.cpp, which
-compiles the device code unchanged and emulates each lane with a host thread.  Checked line by
+"""The persistent z passes (passz_rhs_kernel, passz_rhs2_kernel and passz_full_kernel, the fused
+c2r -> u x w -> r2c of NonlinearRHSBatch, Solver/src/solver.c:656-683) run on the CPU through
+tests/zpass_emu.cpp, which compiles the device code unchanged and runs each lane as a coroutine that
+yields at every wavefront barrier and shuffle (a kernel whose LDS exchanges lacked one would
+disagree with the reference here).  Checked line by
 line against numpy: c2r_z of the six fields (only kz <= N/3 read, Im X[0] and X[N/2] ignored, as
 irfft), the cross product u x w, r2c_z of the three products kept for kz <= N/3 and zero up to the
 padded extent, and the running max of |u_x|+|u_y|+|u_z| per box.  Every (precision, N, elements
@@ -23,11 +25,11 @@ HEADERS = [os.path.join(ROOT, "3d_navier_stokes_amd", "csrc", f) for f in ("nsg_
 @pytest.fixture(scope="module")
 def emu(tmp_path_factory):
     so = str(tmp_path_factory.mktemp("zemu") / "libzemu.so")
-    subprocess.run(["g++", "-std=c++20", "-O1", "-fPIC", "-shared", "-pthread", "-I" + os.path.join(ROOT, "tests", "emu"),
+    subprocess.run(["g++", "-std=c++20", "-O1", "-fPIC", "-shared", "-I" + os.path.join(ROOT, "tests", "emu"),
                     "-o", so, EMU_SRC], check=True, capture_output=True)
     lib = ctypes.CDLL(so)
     vp, i = ctypes.c_void_p, ctypes.c_int
-    lib.emu_passz_rhs.argtypes = [i, i, i, i, i, vp, vp, ctypes.c_double, vp]
+    lib.emu_passz_rhs.argtypes = [i, i, i, i, i, i, vp, vp, ctypes.c_double, vp]
     lib.emu_passz_rhs.restype = i
     lib.emu_extents.argtypes = [i, vp, vp]
     return lib
@@ -54,11 +56,20 @@ def reference(a2, N, kmax, nzp):
 # schedule 1), per precision; (16, 8) in FP64 is the removed instance
 CASES = [("fp64", 8, 4), ("fp64", 16, 4), ("fp64", 16, 8), ("fp64", 32, 8), ("fp64", 32, 4), ("fp64", 64, 8),
          ("fp64", 64, 4), ("fp64", 128, 8), ("fp32", 8, 4), ("fp32", 16, 8), ("fp32", 32, 8), ("fp32", 32, 4),
-         ("fp32", 64, 8), ("fp32", 64, 4), ("fp32", 128, 8)]
+         ("fp32", 64, 8), ("fp32", 64, 4), ("fp32", 128, 8), ("fp32", 128, 16), ("fp64", 128, 16)]
 
 
-@pytest.mark.parametrize("prec,N,ept", CASES)
-def test_zpass_matches_numpy(emu, prec, N, ept):
+# kernel versions: 1 passz_rhs_kernel; 2 passz_rhs2_kernel with base twiddles; 3 with the twiddle
+# table (elements per thread of the N/2-point transform); 4 passz_full_kernel with base twiddles, 5
+# with the twiddle table (elements per thread of the N-point transform: FULL lists them, including
+# one line per wavefront, N/EPT = 64, the geometry of the 512^3 instances)
+FULL = [("fp64", 8, 2), ("fp32", 8, 4), ("fp64", 16, 4), ("fp32", 16, 2), ("fp64", 32, 8), ("fp32", 32, 4),
+        ("fp64", 64, 8), ("fp32", 64, 4), ("fp32", 128, 8), ("fp64", 128, 2), ("fp32", 128, 2), ("fp64", 128, 16)]
+RUNS = [(p, n, e, v) for (p, n, e) in CASES for v in (1, 2, 3)] + [(p, n, e, v) for (p, n, e) in FULL for v in (4, 5)]
+
+
+@pytest.mark.parametrize("prec,N,ept,ver", RUNS)
+def test_zpass_matches_numpy(emu, prec, N, ept, ver):
     kmax, nzp = ctypes.c_int(), ctypes.c_int()
     emu.emu_extents(N, ctypes.byref(kmax), ctypes.byref(nzp))
     kmax, nzp = kmax.value, nzp.value
@@ -71,11 +82,11 @@ def test_zpass_matches_numpy(emu, prec, N, ept):
     inp = np.ascontiguousarray(a2.astype(cdt))
     out = np.full((nbox, 3, N, N, nzp), np.nan, dtype=cdt)  # every element must be written
     got_max = np.zeros(nbox)
-    lines_per_wave = 64 // (N // 2 // ept)
+    lines_per_wave = 64 // ((N // 2 if ver <= 3 else N) // ept)
     groups = (N * N * nbox) // lines_per_wave
     blocks = max(1, min(3, groups // 4 - 1))  # fewer blocks than groups: persistent loop
     scale = 1.0 / N ** 3
-    rc = emu.emu_passz_rhs(N, int(fp32), ept, nbox, blocks, inp.ctypes.data, out.ctypes.data, scale,
+    rc = emu.emu_passz_rhs(N, int(fp32), ept, ver, nbox, blocks, inp.ctypes.data, out.ctypes.data, scale,
                            got_max.ctypes.data)
     assert rc == 0
     assert np.all(np.isfinite(out))

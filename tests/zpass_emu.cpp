@@ -1,12 +1,10 @@
-// CPU emulation of the persistent z pass (test infrastructure): compiles passz_rhs_kernel from
+// CPU emulation of the persistent z passes (test infrastructure): compiles the z-pass kernels from
 // 3d_navier_stokes_amd/csrc/nsg_kernels.hpp with g++ against tests/emu/hip/hip_runtime.h and runs
-// it with one host thread per lane, one block at a time (see that header for the lockstep rules).
+// it with one coroutine per lane, one wavefront at a time (see that header for the lockstep rules).
 // tests/test_zpass_emulation.py loads this as a shared library and checks the result line by line
 // against numpy's irfft / rfft, for every (precision, N, elements per thread) the library builds
 // and for the FP64 N = 16, 8-elements-per-thread instance that the library no longer builds.
 #include <cmath>
-#include <memory>
-#include <thread>
 #include <vector>
 
 #include "../3d_navier_stokes_amd/csrc/nsg_kernels.hpp"
@@ -15,30 +13,15 @@ using namespace nsg;
 
 namespace {
 
+// a grid of `blocks` blocks of `threads` threads: each block's wavefronts run one after another
 template <typename F> void launch(F kern, const PZArgs& a, unsigned blocks, unsigned threads) {
+    static emu::Wave wave;
     for (unsigned b = 0; b < blocks; ++b) {
-        const unsigned nw = (threads + 63) / 64;
-        std::vector<emu::WaveCtx> waves(nw);
-        std::vector<std::unique_ptr<std::barrier<>>> wbar;
-        for (unsigned w = 0; w < nw; ++w) {
-            const unsigned lanes = std::min(64u, threads - 64 * w);
-            wbar.emplace_back(new std::barrier<>((std::ptrdiff_t)lanes));
-            waves[w].bar = wbar.back().get();
-        }
-        std::barrier<> blockbar((std::ptrdiff_t)threads);
-        std::vector<std::thread> th;
-        th.reserve(threads);
-        for (unsigned t = 0; t < threads; ++t)
-            th.emplace_back([&, t, b] {
-                emu::t_tid = {t, 0, 0};
-                emu::t_bid = {b, 0, 0};
-                emu::t_gdim = {blocks, 1, 1};
-                emu::t_bdim = {threads, 1, 1};
-                emu::t_wave = &waves[t / 64];
-                emu::t_block = &blockbar;
-                kern(a);
-            });
-        for (auto& x : th) x.join();
+        emu::t_bid = {b, 0, 0};
+        emu::t_gdim = {blocks, 1, 1};
+        emu::t_bdim = {threads, 1, 1};
+        for (unsigned w0 = 0; w0 < threads; w0 += 64)
+            emu::run_wave(wave, (int)w0, (int)std::min(64u, threads - w0), [&] { kern(a); });
     }
 }
 
@@ -52,8 +35,11 @@ template <typename T> std::vector<cpx_t<T>> table(int L) {
     return t;
 }
 
+// ver 1: passz_rhs_kernel; 2: passz_rhs2_kernel (base twiddles); 3: passz_rhs2_kernel (twiddle table);
+// 4: passz_full_kernel (base twiddles); 5: passz_full_kernel (twiddle table).  eptz is the elements per
+// thread of that kernel's transform (N/2 points for 1-3, N points for 4-5).
 template <typename T, int N, int EPTZ>
-int run(int nbox, unsigned blocks, const void* in, void* out, double scale, double* maxout) {
+int run(int ver, int nbox, unsigned blocks, const void* in, void* out, double scale, double* maxout) {
     constexpr int WPB = 4;
     auto twM = table<T>(N / 2);
     auto twN = table<T>(N);
@@ -73,18 +59,37 @@ int run(int nbox, unsigned blocks, const void* in, void* out, double scale, doub
     a.kmax = Ext<N>::kmax;
     a.NZP = Ext<N>::NZP;
     a.track_max = 1;
-    launch(passz_rhs_kernel<T, N, EPTZ, WPB, 1>, a, blocks, 64 * WPB);
+    constexpr bool half_ok = (N / 2) % EPTZ == 0 && (N / 2) / EPTZ <= 64 && 64 % ((N / 2) / EPTZ) == 0;
+    constexpr bool full_ok = N % EPTZ == 0 && N / EPTZ <= 64 && 64 % (N / EPTZ) == 0;
+    if (ver <= 3) {
+        if constexpr (half_ok) {
+            if (ver == 1) launch(passz_rhs_kernel<T, N, EPTZ, WPB, 1>, a, blocks, 64 * WPB);
+            else if (ver == 2) launch(passz_rhs2_kernel<T, N, EPTZ, WPB, 1, false>, a, blocks, 64 * WPB);
+            else launch(passz_rhs2_kernel<T, N, EPTZ, WPB, 1, true>, a, blocks, 64 * WPB);
+        } else {
+            return -1;
+        }
+    } else if (ver <= 5) {
+        if constexpr (full_ok) {
+            if (ver == 4) launch(passz_full_kernel<T, N, EPTZ, WPB, 1, false>, a, blocks, 64 * WPB);
+            else launch(passz_full_kernel<T, N, EPTZ, WPB, 1, true>, a, blocks, 64 * WPB);
+        } else {
+            return -1;
+        }
+    } else {
+        return -1;
+    }
     for (int b = 0; b < nbox; ++b) maxout[b] = __longlong_as_double((long long)sc[b].maxbits);
     return 0;
 }
 
-template <typename T, int N> int by_ept(int eptz, int nbox, unsigned blocks, const void* in, void* out, double scale,
-                                        double* mx) {
-    constexpr int M = N / 2;
+template <typename T, int N> int by_ept(int ver, int eptz, int nbox, unsigned blocks, const void* in, void* out,
+                                        double scale, double* mx) {
     switch (eptz) {
-        case 2: if constexpr (M / 2 <= 64 && M >= 2) return run<T, N, 2>(nbox, blocks, in, out, scale, mx); break;
-        case 4: if constexpr (M / 4 <= 64 && M >= 4) return run<T, N, 4>(nbox, blocks, in, out, scale, mx); break;
-        case 8: if constexpr (M / 8 <= 64 && M >= 8) return run<T, N, 8>(nbox, blocks, in, out, scale, mx); break;
+        case 2: return run<T, N, 2>(ver, nbox, blocks, in, out, scale, mx);
+        case 4: return run<T, N, 4>(ver, nbox, blocks, in, out, scale, mx);
+        case 8: return run<T, N, 8>(ver, nbox, blocks, in, out, scale, mx);
+        case 16: if constexpr (N >= 16) return run<T, N, 16>(ver, nbox, blocks, in, out, scale, mx); break;
         default: break;
     }
     return -1;
@@ -99,16 +104,16 @@ void emu_extents(int N, int* kmax, int* nzp) {
     *nzp = ((N / 3 + 1 + 15) / 16) * 16;
 }
 // z pass of nbox boxes: in [nbox][6][N][N][NZP], out [nbox][3][N][N][NZP] (complex, fp64 or fp32)
-int emu_passz_rhs(int N, int fp32, int eptz, int nbox, int blocks, const void* in, void* out, double scale,
+int emu_passz_rhs(int N, int fp32, int eptz, int ver, int nbox, int blocks, const void* in, void* out, double scale,
                   double* max_per_box) {
     auto go = [&](auto tag) -> int {
         using T = decltype(tag);
         switch (N) {
-            case 8: return by_ept<T, 8>(eptz, nbox, blocks, in, out, scale, max_per_box);
-            case 16: return by_ept<T, 16>(eptz, nbox, blocks, in, out, scale, max_per_box);
-            case 32: return by_ept<T, 32>(eptz, nbox, blocks, in, out, scale, max_per_box);
-            case 64: return by_ept<T, 64>(eptz, nbox, blocks, in, out, scale, max_per_box);
-            case 128: return by_ept<T, 128>(eptz, nbox, blocks, in, out, scale, max_per_box);
+            case 8: return by_ept<T, 8>(ver, eptz, nbox, blocks, in, out, scale, max_per_box);
+            case 16: return by_ept<T, 16>(ver, eptz, nbox, blocks, in, out, scale, max_per_box);
+            case 32: return by_ept<T, 32>(ver, eptz, nbox, blocks, in, out, scale, max_per_box);
+            case 64: return by_ept<T, 64>(ver, eptz, nbox, blocks, in, out, scale, max_per_box);
+            case 128: return by_ept<T, 128>(ver, eptz, nbox, blocks, in, out, scale, max_per_box);
             default: return -1;
         }
     };
