@@ -79,6 +79,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.nsg_step.argtypes = [vp, ctypes.c_long]
     lib.nsg_dp_stats.argtypes = [vp, P(ctypes.c_longlong), P(ctypes.c_longlong), dp]
     lib.nsg_measures.argtypes = [vp, P(_Totals), vp, vp, ctypes.c_int, ctypes.c_int]
+    lib.nsg_measures_noalloc.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
     lib.nsg_realspace.argtypes = [vp, ctypes.c_int, vp]
     lib.nsg_get_time.argtypes = [vp, dp, dp, dp]
     lib.nsg_set_time.argtypes = [vp, ctypes.c_double, ctypes.c_double]

@@ -509,6 +509,7 @@ static int create_batch(const nsg_params* p, int nbox, nsg_ctx** out) {
     hipError_t e = hipSetDevice(p[0].device);
     if (e != hipSuccess) return fail(c, NSG_ERR_DEVICE, "hipSetDevice(%d): %s", p[0].device, hipGetErrorString(e));
     HIPCHK(c, hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+    nsgi_noalloc_setup(p[0].device);
     {
         hipDeviceProp_t prop;
         HIPCHK(c, hipGetDeviceProperties(&prop, p[0].device));

@@ -93,6 +93,9 @@ int nsgi_r2c(nsg_ctx* c, const double* src, double2* dst);
 // when soa) left in the context's scratch B1; *field points at it until the context's next
 // operation
 int nsgi_realspace(nsg_ctx* c, const double** field, int soa);
+// implemented in nsg_noalloc.hip: per-device set-up of nsg_measures_noalloc's static scratch (the
+// device address of that array), once per device, with `device` current
+void nsgi_noalloc_setup(int device);
 // implemented in nsg_closure.hip: evaluate the closure term of box 0's state into c->closure
 int nsgi_closure_eval(nsg_ctx* c);
 void nsgi_closure_free(nsg_ctx* c);

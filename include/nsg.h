@@ -106,6 +106,17 @@ int nsg_dp_stats(nsg_ctx* ctx, long long* accepted, long long* rejected, double*
  * doubles. quirk = 1 reproduces the reference's operator-precedence (D3) and conjugate-free
  * helicity (D14) arithmetic. */
 int nsg_measures(nsg_ctx* ctx, nsg_totals* out, double* E_k, double* Ens_k, int n_spect, int quirk);
+/* nsg_measures without allocation.  No counterpart in the reference: it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same kernels and same results as nsg_measures (bit for bit), but the per-shell partial sums live
+ * in a static device array of the library (64 KiB per device, 8192 doubles, held until the process
+ * ends; 7 doubles per spectral shell, i.e. n_spect <= 1170, which every supported N satisfies),
+ * and the results go to the caller's DEVICE buffer out[5 + 2 n_spect]: the five totals in
+ * nsg_totals order, then E_k[n_spect], then Ens_k[n_spect].  Asynchronous on the context's stream
+ * (visible after any synchronising call on the context, or hipDeviceSynchronize); no host wait,
+ * no non-finite check.  Calls that overlap on one GPU share that array and race: serialise them
+ * (contexts on one device with concurrent streams must not call it at the same time). */
+int nsg_measures_noalloc(nsg_ctx* ctx, double* out, int n_spect, int quirk);
 /* real-space u (which=0) or w (which=1), scaled 1/N^3 (Solver/src/hdf5_funcs.c:221-223), AoS [N][N][N][3] */
 int nsg_realspace(nsg_ctx* ctx, int which, double* out);
 /* simulation time, current dt, and max(|u_x|+|u_y|+|u_z|) of the last completed step */
