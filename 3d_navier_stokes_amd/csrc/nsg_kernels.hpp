@@ -838,9 +838,9 @@ __global__ void __launch_bounds__(64 * WPB, OCC) passz_rhs2_kernel(PZArgs a) {
 // P[k] = (Z[k] + conj Z[N - k]) / 2, Q[k] = (Z[k] - conj Z[N - k]) / (2 i).  Per line: three
 // inverse transforms (u_y, w_z), (u_z, w_y), (u_x, w_x) -- each pair's product terms are formed as
 // soon as it arrives -- and two forward transforms, (P_x, P_y) and P_z alone.  Against the
-// half-length form this removes the pre/post rotation of every field and its LDS exchange: 5
-// transforms of N points and 10 exchanges per line instead of 9 of N/2 points and 27 (radix 8 at
-// N = 512).  Only kz <= kmax is nonzero on input, so positions kmax < p < N - kmax are zero; the
+// half-length form this removes the pre/post rotation of every field and its LDS exchange: 4.5
+// transforms of N points and 9 exchanges per line instead of 9 of N/2 points and 27 (radix 8 at
+// N = 512); the P_z of two consecutive line groups of a wavefront share one forward transform.  Only kz <= kmax is nonzero on input, so positions kmax < p < N - kmax are zero; the
 // element slots that are zero for every thread are compile-time zeros, and outputs beyond kmax
 // are never formed.  The mirrored coefficients X[N - p] are read from global memory (the line was
 // just fetched, so they come from cache); the separation partner Z[N - k] comes from the thread
@@ -890,7 +890,7 @@ template <int N, int EPT> constexpr int zf_mout() { return Ext<N>::kmax / (N / E
 template <typename T, int N, int EPT, int SY, bool TAB, bool PAIR>
 __device__ __forceinline__ void zf_forward_store(cpx_t<T> (&z)[EPT], cpx_t<T>* out1, cpx_t<T>* out2, int t, int lane,
                                                  cpx_t<T>* lds, int base, const cpx_t<T> (&wz)[zntw2<N, EPT, TAB>()],
-                                                 bool live) {
+                                                 bool live, bool live2) {
     using C = cpx_t<T>;
     using R = decltype(C::x);
     constexpr int TPF = N / EPT, kmax = Ext<N>::kmax, NZP = Ext<N>::NZP;
@@ -914,20 +914,23 @@ __device__ __forceinline__ void zf_forward_store(cpx_t<T> (&z)[EPT], cpx_t<T>* o
 #pragma unroll
         for (int m = 0; m < MOUT; ++m) o1[m] = z[m];
     }
-    if (!live) return;
 #pragma unroll
     for (int m = 0; m < MST; ++m) {
         const int k = t + TPF * m;
         if (k < NZP) {
             const bool in = (m < MOUT) && k <= kmax;
-            out1[k] = in ? o1[m < MOUT ? m : 0] : cmk<C>(0, 0);
-            if constexpr (PAIR) out2[k] = in ? o2[m < MOUT ? m : 0] : cmk<C>(0, 0);
+            if (live) out1[k] = in ? o1[m < MOUT ? m : 0] : cmk<C>(0, 0);
+            if constexpr (PAIR) {
+                if (live2) out2[k] = in ? o2[m < MOUT ? m : 0] : cmk<C>(0, 0);
+            }
         }
     }
     if constexpr (NZP > N) {
         for (int k = N + t; k < NZP; k += TPF) {
-            out1[k] = cmk<C>(0, 0);
-            if constexpr (PAIR) out2[k] = cmk<C>(0, 0);
+            if (live) out1[k] = cmk<C>(0, 0);
+            if constexpr (PAIR) {
+                if (live2) out2[k] = cmk<C>(0, 0);
+            }
         }
     }
 }
@@ -942,7 +945,10 @@ __global__ void __launch_bounds__(64 * WPB, OCC) passz_full_kernel(PZArgs a) {
     constexpr int SY = SYNC_WAVE;
     constexpr int MP = fft_lds_len(N, EPT, true);
     __shared__ C lds[WPB * LPW * MP];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // the wavefront index is uniform across the wavefront: held in a scalar register, so with one
+    // line per wavefront (LPW = 1) every line address is scalar and the loads and stores take a
+    // scalar base plus a per-thread offset that does not change from line to line
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int sub = lane / TPF, t = lane % TPF;
     C wz[zntw2<N, EPT, TAB>()];
     if constexpr (TAB) fft_load_twiddle_table<N, EPT>(wz, t, (const C*)a.twN);
@@ -959,19 +965,24 @@ __global__ void __launch_bounds__(64 * WPB, OCC) passz_full_kernel(PZArgs a) {
         const long long b = gl / nlines;
         return b * stride + (gl - b * nlines) * NZP;
     };
-    double mx = 0.0;
+    // running max of |u_x| + |u_y| + |u_z| in the transform precision (NaN propagates)
+    using R = decltype(C::x);
+    R mx = R(0);
     int mx_box = -1;
     auto flush_max = [&]() {
         if (a.track_max && mx_box >= 0) {
-            double v = mx * a.scale;
+            double v = (double)mx * a.scale;
             for (int off = 32; off > 0; off >>= 1) v = nanmax(v, __shfl_xor(v, off));
             if (lane == 0) atomicMax(&a.sc[mx_box].maxbits, (unsigned long long)__double_as_longlong(v));
         }
-        mx = 0.0;
+        mx = R(0);
     };
 
     long long g = (long long)blockIdx.x * WPB + wv;
     C cu[EPT], cw[EPT];  // raw coefficients of the next pair
+    R pzh[EPT];          // P_z of a line group waiting for its partner
+    C* pz_out = nullptr;
+    bool pz_held = false, pz_live = false;
     if (PF && g < ngroups) {
         long long gl = g * LPW + sub;
         if (gl >= total) gl = total - 1;
@@ -1017,20 +1028,45 @@ __global__ void __launch_bounds__(64 * WPB, OCC) passz_full_kernel(PZArgs a) {
         }
         zfft<N, EPT, +1, SY, TAB>(a3, t, lds, base, wz);
         // a1 = (u_y, w_z), a2 = (u_z, w_y), a3 = (u_x, w_x) at the grid points t + TPF m
-        C pq[EPT], pz[EPT];
+        C pq[EPT];
+        R pz[EPT];
 #pragma unroll
         for (int m = 0; m < EPT; ++m) {
             const auto uy = a1[m].x, wzv = a1[m].y, uz = a2[m].x, wy = a2[m].y, ux = a3[m].x, wx = a3[m].y;
             pq[m] = cmk<C>(uy * wzv - uz * wy, uz * wx - ux * wzv);  // P_x + i P_y
-            pz[m] = cmk<C>(ux * wy - uy * wx, 0);
+            pz[m] = ux * wy - uy * wx;
         }
         if (a.track_max && live) {
 #pragma unroll
-            for (int m = 0; m < EPT; ++m)
-                mx = nanmax(mx, fabs((double)a3[m].x) + fabs((double)a1[m].x) + fabs((double)a2[m].x));
+            for (int m = 0; m < EPT; ++m) {
+                const R v = fabs(a3[m].x) + fabs(a1[m].x) + fabs(a2[m].x);
+                mx = (v >= mx || v != v) ? v : mx;
+            }
         }
-        zf_forward_store<T, N, EPT, SY, TAB, true>(pq, out + 0 * fstride, out + 1 * fstride, t, lane, lds, base, wz, live);
-        zf_forward_store<T, N, EPT, SY, TAB, false>(pz, out + 2 * fstride, nullptr, t, lane, lds, base, wz, live);
+        zf_forward_store<T, N, EPT, SY, TAB, true>(pq, out + 0 * fstride, out + 1 * fstride, t, lane, lds, base, wz, live,
+                                                   live);
+        // P_z of this line group goes into the imaginary part of the previous group's P_z
+        // transform (one forward transform for two lines); an unpaired one is transformed alone
+        if (!pz_held) {
+#pragma unroll
+            for (int m = 0; m < EPT; ++m) pzh[m] = pz[m];
+            pz_out = out + 2 * fstride;
+            pz_live = live;
+            pz_held = true;
+        } else {
+            C zz[EPT];
+#pragma unroll
+            for (int m = 0; m < EPT; ++m) zz[m] = cmk<C>(pzh[m], pz[m]);
+            zf_forward_store<T, N, EPT, SY, TAB, true>(zz, pz_out, out + 2 * fstride, t, lane, lds, base, wz, pz_live,
+                                                       live);
+            pz_held = false;
+        }
+    }
+    if (pz_held) {
+        C zz[EPT];
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) zz[m] = cmk<C>(pzh[m], R(0));
+        zf_forward_store<T, N, EPT, SY, TAB, false>(zz, pz_out, nullptr, t, lane, lds, base, wz, pz_live, false);
     }
     flush_max();
 }

@@ -9,8 +9,9 @@ start (Philox seed 2, 1/2<|u|^2> = 0.5).  The state is resident in HBM before ti
 Prints one JSON line.  `value` is FP32-bulk steps/s; the FP64 plain path is measured the same way
 and reported under "fp64".  `roofline` describes the kernel with the largest share of step time,
 timed with HIP events on the library's own stream.  `cpu_baseline` times the FP64 CPU oracle
-(a C restatement of the reference algorithm, OpenMP) on a bounded sample: one nonlinear-term
-evaluation at 512^3, scaled to steps/s as 1/(4 x that time).
+(a C restatement of the reference algorithm, OpenMP) on a bounded sample: one full RK4-CN step of
+the same forced workload at 512^3.  `--cpu-plan` runs the whole CPU plan instead (256^3 and 512^3
+median of 3 steps after 1 warm-up, and the Taylor-Green configuration end to end).
 
 "ensemble_96_boxes" reports the batched-ensemble workload: 96 independent FP32-bulk boxes (32 at
 32^3, 32 at 48^3, 24 at 64^3, 8 at 96^3), 100 RK4 steps, autotuned FFT schedules, one HIP graph
@@ -281,27 +282,86 @@ def run_closure(nsg, dns, r_dns, Nl=64, steps=200, warmup=20):
     return out
 
 
+def _cpu_host():
+    """host description for the CPU baseline: threads used, logical CPUs, CPU model"""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return os.cpu_count() or 1, model
+
+
+def _oracle_step_time(O, uh, dt, nsteps=1):
+    t0 = time.perf_counter()
+    O.step(uh, 1e-3, dt, 1, "rk4cn", 0.1, 2.5, nsteps=nsteps)
+    return time.perf_counter() - t0
+
+
 def cpu_baseline(N):
+    """Bounded sample of the BASELINE CPU plan: one full FP64 RK4-CN step (4 nonlinear terms, stage
+    combinations, forcing, Crank-Nicolson update) of the forced workload at N^3 on the oracle (the
+    C restatement of the reference algorithm; OpenMP; its own batched radix-4/3/2 FFT, no FFTW in the
+    image), after a 64^3 warm-up step."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O  # the checker; timed here only as the reported CPU baseline
 
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     O.set_threads(threads)
+    _oracle_step_time(O, O.ic_k4(64, k0=2.0, energy=0.5, seed=2), 1e-3)
     uh = O.ic_k4(N, k0=2.0, energy=0.5, seed=2)
-    t0 = time.perf_counter()
-    O.rhs(uh, force_eps=0.1, force_kf=2.5)
-    el = time.perf_counter() - t0
+    el = _oracle_step_time(O, uh, O.cfl_dt(uh, 0.5))
+    del uh
     # the post-processing statistics restatement (sequential, one core) on a 128^3 snapshot
     n_st = 128
     u = O.realspace(O.ic_k4(n_st, 3.0, 0.5, 5), 0)
     t1 = time.perf_counter()
     O.incr_moments(u)
     el_st = time.perf_counter() - t1
-    return {"value": 1.0 / (4.0 * el), "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": f"one nonlinear-term evaluation (9 FP64 3-D real FFTs + products) at {N}^3 took {el:.2f} s; "
-                      f"steps/s = 1/(4 x that), stage combinations not counted",
+    ncpu, model = _cpu_host()
+    return {"value": 1.0 / el, "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"one full FP64 RK4-CN step of the forced workload at {N}^3 (4 nonlinear terms = 36 3-D real "
+                      f"FFTs, stage combinations, forcing, update) took {el:.2f} s on {threads} threads "
+                      f"({ncpu} logical CPUs, {model}); own batched radix-4/3/2 FFT (no FFTW in the image)",
             "increment_stats_moments": {"points_per_s": n_st ** 3 / el_st, "cores": 1,
                                         "sample": f"one moments pass over a {n_st}^3 snapshot took {el_st:.3f} s"}}
+
+
+def cpu_plan():
+    """The whole CPU-baseline plan: FP64 RK4-CN wall time per step at 256^3 and 512^3 (median of 3
+    after 1 warm-up step, the forced workload) and the Taylor-Green configuration end to end (64^3,
+    nu = 1/1600, dt = 1e-3, 200 steps with the measures of the 21 saves)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    O.set_threads(threads)
+    out = {}
+    for N in (256, 512):
+        uh = O.ic_k4(N, k0=2.0, energy=0.5, seed=2)
+        dt = O.cfl_dt(uh, 0.5)
+        _oracle_step_time(O, uh, dt)
+        ts = sorted(_oracle_step_time(O, uh, dt) for _ in range(3))
+        out[f"rk4cn_fp64_{N}"] = {"s_per_step_median": ts[1], "s_per_step": ts, "steps_per_s": 1.0 / ts[1]}
+        log(f"[bench] cpu plan {N}^3: {ts[1]:.2f} s/step (median of 3)")
+        del uh
+    t0 = time.perf_counter()
+    u = O.ic_taylor_green(64)
+    O.measures(u, 1.0 / 1600.0)
+    for _ in range(20):
+        u = O.step(u, 1.0 / 1600.0, 1e-3, nsteps=10)
+        O.measures(u, 1.0 / 1600.0)
+    el = time.perf_counter() - t0
+    out["taylor_green_64_200_steps"] = {"s": el, "steps_per_s": 200.0 / el}
+    log(f"[bench] cpu plan Taylor-Green 64^3 200 steps: {el:.2f} s")
+    ncpu, model = _cpu_host()
+    out["host"] = {"threads": threads, "logical_cpus": ncpu, "cpu": model,
+                   "fft": "oracle's own batched radix-4/3/2 transform (FFTW absent)"}
+    return out
 
 
 def main():
@@ -313,6 +373,8 @@ def main():
     ap.add_argument("--prof-iters", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=512)
+    ap.add_argument("--cpu-plan", action="store_true",
+                    help="only run the whole CPU-baseline plan (256^3/512^3 steps, Taylor-Green end to end)")
     ap.add_argument("--ensemble-steps", type=int, default=100, help="0 skips the 96-box ensemble")
     ap.add_argument("--no-closure", dest="closure", action="store_false", help="skip the closure-LES timing")
     ap.add_argument("--no-incr-stats", dest="incr_stats", action="store_false",
@@ -324,6 +386,9 @@ def main():
     ap.add_argument("--incr-only", type=int, default=0,
                     help="only time the increment statistics of a k^4 start field of this size")
     args = ap.parse_args()
+    if args.cpu_plan:
+        print(json.dumps({"cpu_plan": cpu_plan()}), flush=True)
+        return
     if args.incr_only:
         nsg = load_nsg()
         with nsg.Solver(args.incr_only, precision="fp32bulk") as s:

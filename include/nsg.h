@@ -142,10 +142,9 @@ int nsg_schedule_count(int N);
 int nsg_set_schedule(nsg_ctx* ctx, int sched);
 int nsg_autotune(nsg_ctx* ctx, int iters, int use_cache, int* chosen, double* ms_per_step);
 
-/* Ensembles: many independent boxes on one GPU.  The reference runs one box per process and
- * sweeps parameters with one solver invocation per run (RunScripts/run_stats*.sh call the
- * binaries once per case); an ensemble replaces that loop of SpectralSolve runs
- * (Solver/src/solver.c:50-207).  Boxes sharing N, precision, scheme, visc_pow, forcing and device
+/* Ensembles: many independent boxes on one GPU.  The reference integrates one box per process,
+ * so a parameter sweep is one solver run per case; an ensemble replaces that loop of SpectralSolve
+ * runs (Solver/src/solver.c:50-207).  Boxes sharing N, precision, scheme, visc_pow, forcing and device
  * form one batch (one launch per pass covers the whole batch); nu, dt, t0, cfl and max_speed are
  * per box.  nsg_ensemble_step replays a HIP graph holding one step of every batch, the batches
  * on their own streams; each box evolves exactly as it would alone under the same schedule.

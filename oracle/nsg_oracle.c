@@ -145,80 +145,216 @@ void oracle_fft1d(int n, double* data, int sign) {
 }
 
 /* ------------------------------------------------------------------------------------------ */
+/* Batched transform for the 3-D passes: B lines of length n stored interleaved, element e of    */
+/* line b at a[e B + b], so every butterfly runs over B contiguous values.  Self-sorting         */
+/* (Stockham) decimation in frequency, radix 4, 3, 2 for n = 2^a 3^b (the supported grids):      */
+/* stage of radix r on a sub-length L = r m at stride s (s = B times the radices already done):  */
+/*   y[q + s (r p + k)] = W_L^{k p} sum_j W_r^{j k} x[q + s (p + j m)],  p < m, q < s,           */
+/* then the same on L/r with stride r s, ping-ponging between the two buffers.  Other n use the  */
+/* recursive transform above line by line.                                                      */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct {
+    int n, B, nst, ok;
+    int radix[32];
+    cplx* tw[32]; /* per stage: W_L^{k p} for p < m, k < r, at [p r + k] */
+    fft_plan line;
+} bfft_plan;
+
+static void bplan_init(bfft_plan* q, int n, int B) {
+    memset(q, 0, sizeof *q);
+    q->n = n;
+    q->B = B;
+    plan_init(&q->line, n);
+    int m = n;
+    while (m % 4 == 0) { q->radix[q->nst++] = 4; m /= 4; }
+    while (m % 2 == 0) { q->radix[q->nst++] = 2; m /= 2; }
+    while (m % 3 == 0) { q->radix[q->nst++] = 3; m /= 3; }
+    q->ok = (m == 1);
+    if (!q->ok) return;
+    int L = n;
+    for (int st = 0; st < q->nst; ++st) {
+        const int r = q->radix[st], mm = L / r;
+        q->tw[st] = (cplx*)malloc(sizeof(cplx) * (size_t)mm * r);
+        for (int pp = 0; pp < mm; ++pp)
+            for (int k = 0; k < r; ++k) /* W_L^{k p} = W_n^{k p n / L} */
+                q->tw[st][pp * r + k] = q->line.tw[((long)k * pp * (n / L)) % n];
+        L = mm;
+    }
+}
+static void bplan_free(bfft_plan* q) {
+    for (int st = 0; st < q->nst; ++st) free(q->tw[st]);
+    plan_free(&q->line);
+}
+
+static inline cplx cmulc(cplx a, cplx w, int sign) { /* a * w (sign < 0) or a * conj(w) */
+    cplx r;
+    if (sign < 0) { r.re = a.re * w.re - a.im * w.im; r.im = a.re * w.im + a.im * w.re; }
+    else { r.re = a.re * w.re + a.im * w.im; r.im = a.im * w.re - a.re * w.im; }
+    return r;
+}
+
+/* transform the B interleaved lines in a (scratch y of the same size); result in a */
+static void bfft(const bfft_plan* q, cplx* a, cplx* y, int sign) {
+    const int n = q->n, B = q->B;
+    if (!q->ok) { /* line by line through the recursive transform */
+        cplx* line = (cplx*)malloc(sizeof(cplx) * (size_t)n * 2);
+        for (int b = 0; b < B; ++b) {
+            for (int e = 0; e < n; ++e) line[e] = a[(long)e * B + b];
+            fft_line(&q->line, line, line + n, sign);
+            for (int e = 0; e < n; ++e) a[(long)e * B + b] = line[e];
+        }
+        free(line);
+        return;
+    }
+    cplx *x = a, *z = y;
+    long s = B;
+    int L = n;
+    const double h3 = 0.86602540378443864676; /* sin(2 pi / 3) */
+    for (int st = 0; st < q->nst; ++st) {
+        const int r = q->radix[st], m = L / r;
+        const cplx* tw = q->tw[st];
+        for (int pp = 0; pp < m; ++pp) {
+            const cplx* w = tw + (long)pp * r;
+            const cplx* x0 = x + s * pp;
+            cplx* y0 = z + s * ((long)r * pp);
+            if (r == 4) {
+                const cplx *x1 = x0 + s * m, *x2 = x0 + 2 * s * m, *x3 = x0 + 3 * s * m;
+                for (long qq = 0; qq < s; ++qq) {
+                    const cplx A = x0[qq], Bv = x1[qq], Cv = x2[qq], D = x3[qq];
+                    const double apr = A.re + Cv.re, api = A.im + Cv.im, amr = A.re - Cv.re, ami = A.im - Cv.im;
+                    const double bpr = Bv.re + D.re, bpi = Bv.im + D.im, bmr = Bv.re - D.re, bmi = Bv.im - D.im;
+                    /* sign * i (B - D): forward (-1) gives -i(B-D) = (bmi, -bmr) */
+                    const double jr = sign < 0 ? bmi : -bmi, ji = sign < 0 ? -bmr : bmr;
+                    cplx o0 = {apr + bpr, api + bpi}, o1 = {amr + jr, ami + ji};
+                    cplx o2 = {apr - bpr, api - bpi}, o3 = {amr - jr, ami - ji};
+                    y0[qq] = o0;
+                    y0[s + qq] = cmulc(o1, w[1], sign);
+                    y0[2 * s + qq] = cmulc(o2, w[2], sign);
+                    y0[3 * s + qq] = cmulc(o3, w[3], sign);
+                }
+            } else if (r == 2) {
+                const cplx* x1 = x0 + s * m;
+                for (long qq = 0; qq < s; ++qq) {
+                    const cplx A = x0[qq], Bv = x1[qq];
+                    cplx o0 = {A.re + Bv.re, A.im + Bv.im}, o1 = {A.re - Bv.re, A.im - Bv.im};
+                    y0[qq] = o0;
+                    y0[s + qq] = cmulc(o1, w[1], sign);
+                }
+            } else { /* r == 3 */
+                const cplx *x1 = x0 + s * m, *x2 = x0 + 2 * s * m;
+                const double sh = sign < 0 ? -h3 : h3;
+                for (long qq = 0; qq < s; ++qq) {
+                    const cplx A = x0[qq], Bv = x1[qq], Cv = x2[qq];
+                    const double tr = Bv.re + Cv.re, ti = Bv.im + Cv.im, dr = Bv.re - Cv.re, di = Bv.im - Cv.im;
+                    const double mr = A.re - 0.5 * tr, mi = A.im - 0.5 * ti;
+                    /* i sh (B - C) */
+                    const double er = -sh * di, ei = sh * dr;
+                    cplx o0 = {A.re + tr, A.im + ti}, o1 = {mr + er, mi + ei}, o2 = {mr - er, mi - ei};
+                    y0[qq] = o0;
+                    y0[s + qq] = cmulc(o1, w[1], sign);
+                    y0[2 * s + qq] = cmulc(o2, w[2], sign);
+                }
+            }
+        }
+        cplx* t = x;
+        x = z;
+        z = t;
+        s *= r;
+        L = m;
+    }
+    if (x != a) memcpy(a, x, sizeof(cplx) * (size_t)n * B);
+}
+
+/* ------------------------------------------------------------------------------------------ */
 /* 3-D real transforms (single component).  These restate what the six FFTW plans of           */
 /* InitializeFFTWPlans do (Solver/src/solver.c:2034-2073) with D1 corrected: real size N^3.    */
 /* ------------------------------------------------------------------------------------------ */
+#define ZBATCH 16 /* z lines per batched transform */
+
 void oracle_r2c3d(int N, const double* in, double* outd) {
     const int Nzf = N / 2 + 1;
+    const long NN = (long)N * N;
     cplx* out = (cplx*)outd;
-    fft_plan p;
-    plan_init(&p, N);
+    const int tailB = (int)(NN % ZBATCH);
+    bfft_plan pz, pzt, pyx;
+    bplan_init(&pz, N, ZBATCH);
+    bplan_init(&pzt, N, tailB > 0 ? tailB : 1);
+    bplan_init(&pyx, N, Nzf);
 #pragma omp parallel
     {
-        cplx* line = (cplx*)malloc(sizeof(cplx) * (size_t)N);
-        cplx* scr = (cplx*)malloc(sizeof(cplx) * (size_t)N);
+        cplx* buf = (cplx*)malloc(sizeof(cplx) * (size_t)N * (Nzf > ZBATCH ? Nzf : ZBATCH));
+        cplx* scr = (cplx*)malloc(sizeof(cplx) * (size_t)N * (Nzf > ZBATCH ? Nzf : ZBATCH));
 #pragma omp for schedule(static)
-        for (long ij = 0; ij < (long)N * N; ++ij) { /* z: real -> half complex */
-            const double* src = in + ij * N;
-            for (int k = 0; k < N; ++k) { line[k].re = src[k]; line[k].im = 0.0; }
-            fft_line(&p, line, scr, -1);
-            for (int k = 0; k < Nzf; ++k) out[ij * Nzf + k] = line[k];
+        for (long b0 = 0; b0 < NN; b0 += ZBATCH) { /* z: real -> half complex, ZBATCH lines at a time */
+            const int B = (int)(NN - b0 < ZBATCH ? NN - b0 : ZBATCH);
+            for (int e = 0; e < N; ++e)
+                for (int b = 0; b < B; ++b) { buf[e * B + b].re = in[(b0 + b) * N + e]; buf[e * B + b].im = 0.0; }
+            bfft(B == ZBATCH ? &pz : &pzt, buf, scr, -1);
+            for (int b = 0; b < B; ++b)
+                for (int k = 0; k < Nzf; ++k) out[(b0 + b) * Nzf + k] = buf[k * B + b];
         }
 #pragma omp for schedule(static)
-        for (long ik = 0; ik < (long)N * Nzf; ++ik) { /* y */
-            long i = ik / Nzf, k = ik % Nzf;
-            for (int j = 0; j < N; ++j) line[j] = out[(i * N + j) * Nzf + k];
-            fft_line(&p, line, scr, -1);
-            for (int j = 0; j < N; ++j) out[(i * N + j) * Nzf + k] = line[j];
-        }
+        for (int i = 0; i < N; ++i) /* y: the x-plane [N y][Nzf] is Nzf interleaved lines */
+            bfft(&pyx, out + (long)i * N * Nzf, scr, -1);
 #pragma omp for schedule(static)
-        for (long jk = 0; jk < (long)N * Nzf; ++jk) { /* x */
-            for (int i = 0; i < N; ++i) line[i] = out[(long)i * N * Nzf + jk];
-            fft_line(&p, line, scr, -1);
-            for (int i = 0; i < N; ++i) out[(long)i * N * Nzf + jk] = line[i];
+        for (int j = 0; j < N; ++j) { /* x: gather the rows (i, j) into [N x][Nzf] */
+            for (int i = 0; i < N; ++i) memcpy(buf + (long)i * Nzf, out + ((long)i * N + j) * Nzf, sizeof(cplx) * Nzf);
+            bfft(&pyx, buf, scr, -1);
+            for (int i = 0; i < N; ++i) memcpy(out + ((long)i * N + j) * Nzf, buf + (long)i * Nzf, sizeof(cplx) * Nzf);
         }
-        free(line);
+        free(buf);
         free(scr);
     }
-    plan_free(&p);
+    bplan_free(&pz);
+    bplan_free(&pzt);
+    bplan_free(&pyx);
 }
 
 void oracle_c2r3d(int N, const double* ind, double* out) {
     const int Nzf = N / 2 + 1;
-    const long C = (long)N * N * Nzf;
+    const long NN = (long)N * N;
+    const long C = NN * Nzf;
     cplx* w = (cplx*)malloc(sizeof(cplx) * (size_t)C);
     memcpy(w, ind, sizeof(cplx) * (size_t)C);
-    fft_plan p;
-    plan_init(&p, N);
+    const int tailB = (int)(NN % ZBATCH);
+    bfft_plan pz, pzt, pyx;
+    bplan_init(&pz, N, ZBATCH);
+    bplan_init(&pzt, N, tailB > 0 ? tailB : 1);
+    bplan_init(&pyx, N, Nzf);
 #pragma omp parallel
     {
-        cplx* line = (cplx*)malloc(sizeof(cplx) * (size_t)N);
-        cplx* scr = (cplx*)malloc(sizeof(cplx) * (size_t)N);
+        cplx* buf = (cplx*)malloc(sizeof(cplx) * (size_t)N * (Nzf > ZBATCH ? Nzf : ZBATCH));
+        cplx* scr = (cplx*)malloc(sizeof(cplx) * (size_t)N * (Nzf > ZBATCH ? Nzf : ZBATCH));
 #pragma omp for schedule(static)
-        for (long jk = 0; jk < (long)N * Nzf; ++jk) { /* x */
-            for (int i = 0; i < N; ++i) line[i] = w[(long)i * N * Nzf + jk];
-            fft_line(&p, line, scr, +1);
-            for (int i = 0; i < N; ++i) w[(long)i * N * Nzf + jk] = line[i];
+        for (int j = 0; j < N; ++j) { /* x */
+            for (int i = 0; i < N; ++i) memcpy(buf + (long)i * Nzf, w + ((long)i * N + j) * Nzf, sizeof(cplx) * Nzf);
+            bfft(&pyx, buf, scr, +1);
+            for (int i = 0; i < N; ++i) memcpy(w + ((long)i * N + j) * Nzf, buf + (long)i * Nzf, sizeof(cplx) * Nzf);
         }
 #pragma omp for schedule(static)
-        for (long ik = 0; ik < (long)N * Nzf; ++ik) { /* y */
-            long i = ik / Nzf, k = ik % Nzf;
-            for (int j = 0; j < N; ++j) line[j] = w[(i * N + j) * Nzf + k];
-            fft_line(&p, line, scr, +1);
-            for (int j = 0; j < N; ++j) w[(i * N + j) * Nzf + k] = line[j];
-        }
+        for (int i = 0; i < N; ++i) /* y */
+            bfft(&pyx, w + (long)i * N * Nzf, scr, +1);
 #pragma omp for schedule(static)
-        for (long ij = 0; ij < (long)N * N; ++ij) { /* z: Hermitian extension, keep real part */
-            const cplx* src = w + ij * Nzf;
-            for (int k = 0; k < Nzf; ++k) line[k] = src[k];
-            for (int k = 1; k < N - N / 2; ++k) { line[N - k].re = src[k].re; line[N - k].im = -src[k].im; }
-            fft_line(&p, line, scr, +1);
-            for (int k = 0; k < N; ++k) out[ij * N + k] = line[k].re;
+        for (long b0 = 0; b0 < NN; b0 += ZBATCH) { /* z: Hermitian extension, keep the real part */
+            const int B = (int)(NN - b0 < ZBATCH ? NN - b0 : ZBATCH);
+            for (int b = 0; b < B; ++b) {
+                const cplx* src = w + (b0 + b) * Nzf;
+                for (int k = 0; k < Nzf; ++k) buf[k * B + b] = src[k];
+                for (int k = 1; k < N - N / 2; ++k) {
+                    buf[(N - k) * B + b].re = src[k].re;
+                    buf[(N - k) * B + b].im = -src[k].im;
+                }
+            }
+            bfft(B == ZBATCH ? &pz : &pzt, buf, scr, +1);
+            for (int b = 0; b < B; ++b)
+                for (int k = 0; k < N; ++k) out[(b0 + b) * N + k] = buf[k * B + b].re;
         }
-        free(line);
+        free(buf);
         free(scr);
     }
-    plan_free(&p);
+    bplan_free(&pz);
+    bplan_free(&pzt);
+    bplan_free(&pyx);
     free(w);
 }
 

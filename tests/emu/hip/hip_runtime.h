@@ -128,6 +128,8 @@ inline void __syncthreads() {
     std::abort();
 }
 template <typename V> inline V __shfl(V v, int src) { return emu::shfl_from(v, src); }
+// the value of the first lane (every lane of an emulated wavefront is active)
+inline int __builtin_amdgcn_readfirstlane(int v) { return emu::shfl_from(v, 0); }
 template <typename V> inline V __shfl_xor(V v, int mask) { return emu::shfl_from(v, (int)(emu::t_tid.x & 63) ^ mask); }
 inline unsigned long long atomicMax(unsigned long long* p, unsigned long long v) {
     const unsigned long long o = *p;
