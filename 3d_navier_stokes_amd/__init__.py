@@ -446,8 +446,9 @@ class Ensemble:
     def step_serial(self, nsteps: int = 1):
         self._check(self.lib.nsg_ensemble_step_serial(self._h, int(nsteps)))
 
-    def time_steps(self, nsteps: int, serial: bool = False) -> float:
-        """Milliseconds for nsteps steps of every box (HIP events), graph or per-box launches."""
+    def time_steps(self, nsteps: int, serial: int = 0) -> float:
+        """Milliseconds for nsteps steps of every box (HIP events): 0 graph replays, 1 per-box
+        launches, 2 per-batch launches on the batches' streams without a graph."""
         ms = ctypes.c_double()
         self._check(self.lib.nsg_ensemble_time_steps(self._h, int(nsteps), int(serial), ctypes.byref(ms)))
         return ms.value

@@ -249,16 +249,22 @@ template <typename T, int N, int S> static constexpr bool zrhs_persistent() {
 }
 
 // Full-length paired z pass (passz_full_kernel): elements per thread of its N-point transforms, or
-// 0 where the half-length form is kept.  Measured at 512^3 (isolated launches, same inputs): FP64
-// 1.33 ms against 1.65 ms (the half-length form is bound by its 27 LDS exchanges per line); FP32
-// 0.745 ms against 0.84 ms.  At 256^3 both forms take the same time within 5 %; the full form
-// is used for both precisions there as well.  Other sizes keep the half-length form.
+// 0 where the half-length form (passz_rhs_kernel) is kept.  Measured at 512^3 (isolated launches,
+// same inputs): FP64 1.33 ms against 1.65 ms for the half-length form (bound by its 27 LDS
+// exchanges per line), FP32 0.72 ms against 0.84 ms.  The sizes of the ensembles (32..128, and 12,
+// 24, 48, 96 with a radix-3 stage, which had only the block-synchronous half-length form) use it
+// too.  1024 keeps the half-length form (16 elements per thread would not fit the registers).
 template <typename T, int N> static constexpr int zfull_ept() {
-    if constexpr (is64<T>()) return N == 256 ? 4 : N == 512 ? 8 : 0;
-    else return (N == 256 || N == 512) ? 8 : 0;
+    if constexpr (N == 1024) return 0;
+    if constexpr (is64<T>()) return (N % 3 == 0) ? 6 : (N == 512) ? 8 : 4;
+    else return (N == 48 || N == 96) ? 12 : (N % 3 == 0) ? 6 : (N == 8 || N == 16) ? 4 : 8;
 }
-// waves per SIMD the full form is compiled for (no spills at these bounds; FP64 at 512: 256 VGPRs)
-template <typename T, int N> static constexpr int zfull_occ() { return (is64<T>() && N == 512) ? 2 : 3; }
+// waves per SIMD the full form is compiled for (no scratch at these bounds except FP64 96 and 512:
+// 20 and 60 bytes per lane)
+template <typename T, int N> static constexpr int zfull_occ() {
+    if constexpr (is64<T>()) return (N == 32 || N == 64 || N == 256) ? 3 : 2;
+    else return (N == 48 || N == 96) ? 2 : 3;
+}
 
 template <typename T, int N, int S>
 static int launch_passz_rhs(nsg_ctx* c, int b0, int nb, double scale, int track_max) {
@@ -276,24 +282,18 @@ static int launch_passz_rhs(nsg_ctx* c, int b0, int nb, double scale, int track_
     long long blocks = (groups + WPB - 1) / WPB;
     const long long cap = (long long)c->ncu * c->zblocks_per_cu;
     if (blocks > cap) blocks = cap;
+    prof_begin(c);
     if constexpr (EF > 0) {
-        prof_begin(c);
         hipLaunchKernelGGL((passz_full_kernel<T, N, EF, WPB, zfull_occ<T, N>(), true, false>), dim3((unsigned)blocks),
                            dim3(64 * WPB), 0, c->s, a);
-        LAUNCHCHK(c);
-        prof_end(c, 2);
-        return NSG_OK;
-    }
-    prof_begin(c);
-    // Fields of at most 16 VGPRs per thread: compiled for 2 waves per SIMD.  The kernel's natural
-    // allocation sits at 250-256 VGPRs, and a build one register over would fall to 1 wave per
-    // SIMD (measured 1.7x slower at 512^3 FP32).  Larger fields (FP64, 8 per thread) would spill
-    // under that bound and keep the unbounded build.
-    constexpr int OCC = (EPTZ * sizeof(cpx_t<T>) <= 64) ? 2 : 1;
-    if (c->zocc >= 3)
-        hipLaunchKernelGGL((passz_rhs_kernel<T, N, EPTZ, WPB, 3>), dim3((unsigned)blocks), dim3(64 * WPB), 0, c->s, a);
-    else
+    } else {
+        // Fields of at most 16 VGPRs per thread: compiled for 2 waves per SIMD.  The kernel's natural
+        // allocation sits at 250-256 VGPRs, and a build one register over would fall to 1 wave per
+        // SIMD (measured 1.7x slower at 512^3 FP32).  Larger fields (FP64, 8 per thread) would spill
+        // under that bound and keep the unbounded build.
+        constexpr int OCC = (EPTZ * sizeof(cpx_t<T>) <= 64) ? 2 : 1;
         hipLaunchKernelGGL((passz_rhs_kernel<T, N, EPTZ, WPB, OCC>), dim3((unsigned)blocks), dim3(64 * WPB), 0, c->s, a);
+    }
     LAUNCHCHK(c);
     prof_end(c, 2);
     return NSG_OK;
@@ -365,7 +365,7 @@ static int run_stage(nsg_ctx* c, double2* uh, int b0, int nb, int stage, bool tr
     int rc;
     if ((rc = launch_pass1<T, N, S>(c, uh, b0, nb, kprev, stage, 0, force))) return rc;
     if ((rc = launch_passy<T, N, S, +1>(c, b0, nb, 6))) return rc;
-    if constexpr (zrhs_persistent<T, N, S>()) {
+    if constexpr (zfull_ept<T, N>() > 0 || zrhs_persistent<T, N, S>()) {
         if ((rc = launch_passz_rhs<T, N, S>(c, b0, nb, 1.0 / N3, track_max ? 1 : 0))) return rc;
     } else {
         if ((rc = launch_passz<T, N, S, ZM_RHS>(c, (const char*)c->B2 + b0 * c->a2box * esz<T>(),
@@ -516,8 +516,6 @@ static int create_batch(const nsg_params* p, int nbox, nsg_ctx** out) {
         c->ncu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
         const char* zb = std::getenv("NSG_ZBLOCKS_PER_CU");
         if (zb && std::atoi(zb) > 0) c->zblocks_per_cu = std::atoi(zb);
-        const char* zo = std::getenv("NSG_ZOCC");
-        if (zo && std::atoi(zo) > 0) c->zocc = std::atoi(zo);
     }
     HIPCHK(c, hipEventCreate(&c->ev[0]));
     HIPCHK(c, hipEventCreate(&c->ev[1]));
@@ -1051,6 +1049,7 @@ struct nsg_ensemble {
     hipEvent_t ev[2] = {nullptr, nullptr};
     std::string err;
     int device = 0;
+    int gsteps = 1;  // ensemble steps held by the graph (one replay advances every box by gsteps)
 };
 
 static int efail(nsg_ensemble* e, int code, const std::string& msg) {
@@ -1112,8 +1111,24 @@ static void ens_drop_graph(nsg_ensemble* e) {
     e->graph = nullptr;
 }
 
+// n steps of every batch launched directly: the batches fork from the master stream onto their
+// own streams and join back (the same kernels and order per batch as the graph)
+static int ens_direct(nsg_ensemble* e, long n) {
+    EHIP(e, hipEventRecord(e->fork, e->master));
+    for (size_t g = 0; g < e->groups.size(); ++g) {
+        nsg_ctx* c = e->groups[g];
+        EHIP(e, hipStreamWaitEvent(c->s, e->fork, 0));
+        const int rc = enqueue_steps(c, n, 0, c->nbox);
+        if (rc) return egroup(e, c, rc);
+        EHIP(e, hipEventRecord(e->join[g], c->s));
+        EHIP(e, hipStreamWaitEvent(e->master, e->join[g], 0));
+    }
+    return NSG_OK;
+}
+
 static int ens_build_graph(nsg_ensemble* e) {
     if (e->exec) return NSG_OK;
+    if (const char* gs = std::getenv("NSG_ENS_GRAPH_STEPS")) e->gsteps = std::max(1, std::atoi(gs));
     EHIP(e, hipStreamBeginCapture(e->master, hipStreamCaptureModeThreadLocal));
     int rc = NSG_OK;
     std::string msg;
@@ -1124,7 +1139,7 @@ static int ens_build_graph(nsg_ensemble* e) {
         if ((st = hipStreamWaitEvent(c->s, e->fork, 0)) != hipSuccess) {
             rc = NSG_ERR_DEVICE; msg = std::string("hipStreamWaitEvent: ") + hipGetErrorString(st); break;
         }
-        if ((rc = enqueue_steps(c, 1, 0, c->nbox))) { msg = c->err; break; }
+        if ((rc = enqueue_steps(c, e->gsteps, 0, c->nbox))) { msg = c->err; break; }
         if ((st = hipEventRecord(e->join[g], c->s)) != hipSuccess ||
             (st = hipStreamWaitEvent(e->master, e->join[g], 0)) != hipSuccess) {
             rc = NSG_ERR_DEVICE; msg = std::string("join: ") + hipGetErrorString(st); break;
@@ -1153,9 +1168,10 @@ static int ens_check(nsg_ensemble* e) {
 int nsg_ensemble_step(nsg_ensemble* e, long nsteps) {
     if (!e || nsteps < 0) return NSG_ERR_INVALID;
     EHIP(e, hipSetDevice(e->device));
-    const int rc = ens_build_graph(e);
+    int rc = ens_build_graph(e);
     if (rc) return rc;
-    for (long i = 0; i < nsteps; ++i) EHIP(e, hipGraphLaunch(e->exec, e->master));
+    for (long i = 0; i + e->gsteps <= nsteps; i += e->gsteps) EHIP(e, hipGraphLaunch(e->exec, e->master));
+    if (nsteps % e->gsteps && (rc = ens_direct(e, nsteps % e->gsteps))) return rc;
     EHIP(e, hipStreamSynchronize(e->master));
     return ens_check(e);
 }
@@ -1265,10 +1281,12 @@ int nsg_ensemble_time_steps(nsg_ensemble* e, long nsteps, int serial, double* ms
     if (!e || nsteps <= 0 || !ms) return NSG_ERR_INVALID;
     EHIP(e, hipSetDevice(e->device));
     int rc;
-    if (!serial && (rc = ens_build_graph(e))) return rc;
+    if (serial == 0 && (rc = ens_build_graph(e))) return rc;
     for (nsg_ctx* g : e->groups) EHIP(e, hipStreamSynchronize(g->s));
     EHIP(e, hipEventRecord(e->ev[0], e->master));
-    if (serial) {
+    if (serial == 2) {
+        if ((rc = ens_direct(e, nsteps))) return rc;
+    } else if (serial) {
         // the serial path runs on the groups' streams: order them after the start event and
         // bring them back to the master stream before the end event
         for (nsg_ctx* g : e->groups) EHIP(e, hipStreamWaitEvent(g->s, e->ev[0], 0));
@@ -1281,7 +1299,8 @@ int nsg_ensemble_time_steps(nsg_ensemble* e, long nsteps, int serial, double* ms
             EHIP(e, hipStreamWaitEvent(e->master, e->join[gi], 0));
         }
     } else {
-        for (long i = 0; i < nsteps; ++i) EHIP(e, hipGraphLaunch(e->exec, e->master));
+        for (long i = 0; i + e->gsteps <= nsteps; i += e->gsteps) EHIP(e, hipGraphLaunch(e->exec, e->master));
+        if (nsteps % e->gsteps && (rc = ens_direct(e, nsteps % e->gsteps))) return rc;
     }
     EHIP(e, hipEventRecord(e->ev[1], e->master));
     EHIP(e, hipEventSynchronize(e->ev[1]));

@@ -20,7 +20,6 @@ struct nsg_ctx {
     int sched = 0;
     bool fp32 = false, any_cfl = false;
     int ncu = 256, zblocks_per_cu = 8;  // persistent z-pass grid = ncu * zblocks_per_cu blocks
-    int zocc = 1;                       // >= 3: z kernel compiled for 3 waves/SIMD (fewer VGPRs)
     hipStream_t s = nullptr;
     // per-box element strides of the state/K arrays and of A1..A4
     long long sbox = 0, a1box = 0, a2box = 0, a3box = 0, a4box = 0;

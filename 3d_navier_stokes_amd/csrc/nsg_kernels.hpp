@@ -838,14 +838,15 @@ __global__ void __launch_bounds__(64 * WPB, OCC) passz_rhs2_kernel(PZArgs a) {
 // P[k] = (Z[k] + conj Z[N - k]) / 2, Q[k] = (Z[k] - conj Z[N - k]) / (2 i).  Per line: three
 // inverse transforms (u_y, w_z), (u_z, w_y), (u_x, w_x) -- each pair's product terms are formed as
 // soon as it arrives -- and two forward transforms, (P_x, P_y) and P_z alone.  Against the
-// half-length form this removes the pre/post rotation of every field and its LDS exchange: 4.5
-// transforms of N points and 9 exchanges per line instead of 9 of N/2 points and 27 (radix 8 at
-// N = 512); the P_z of two consecutive line groups of a wavefront share one forward transform.  Only kz <= kmax is nonzero on input, so positions kmax < p < N - kmax are zero; the
+// half-length form this removes the pre/post rotation of every field and its LDS exchange: 5
+// transforms of N points and 10 exchanges per line instead of 9 of N/2 points and 27 (radix 8 at
+// N = 512).  Only kz <= kmax is nonzero on input, so positions kmax < p < N - kmax are zero; the
 // element slots that are zero for every thread are compile-time zeros, and outputs beyond kmax
 // are never formed.  The mirrored coefficients X[N - p] are read from global memory (the line was
 // just fetched, so they come from cache); the separation partner Z[N - k] comes from the thread
 // holding position N - k by a cross-lane shuffle (thread TPF - t, element EPT - 1 - m).
-// One line per TPF = N / EPT threads; lines per wavefront LPW = 64 / TPF; persistent loop.
+// One line per TPF = N / EPT threads (a power of two: TPF - t is the partner thread); lines per
+// wavefront LPW = 64 / TPF; persistent loop.  N may carry a factor 3 (radix-3 stages).
 // ------------------------------------------------------------------------------------------
 // raw coefficients of one field for the positions p = t + TPF m of a full-length transform:
 // X[p] for p <= kmax, X[N - p] for p >= N - kmax, zero between (compile-time zero where the whole
@@ -905,7 +906,7 @@ __device__ __forceinline__ void zf_forward_store(cpx_t<T> (&z)[EPT], cpx_t<T>* o
             C zp;  // Z[N - k], k = t + TPF m
             zp.x = __shfl(z[EPT - 1 - m].x, src);
             zp.y = __shfl(z[EPT - 1 - m].y, src);
-            if (t == 0) zp = z[(EPT - m) & (EPT - 1)];
+            if (t == 0) zp = z[(EPT - m) % EPT];
             const C zk = z[m];
             o1[m] = cmk<C>(R(0.5) * (zk.x + zp.x), R(0.5) * (zk.y - zp.y));
             o2[m] = cmk<C>(R(0.5) * (zk.y + zp.y), R(0.5) * (zp.x - zk.x));
@@ -941,7 +942,7 @@ template <typename T, int N, int EPT, int WPB, int OCC, bool TAB, bool PF = true
 __global__ void __launch_bounds__(64 * WPB, OCC) passz_full_kernel(PZArgs a) {
     using C = cpx_t<T>;
     constexpr int TPF = N / EPT, LPW = 64 / TPF;
-    static_assert(TPF <= 64 && 64 % TPF == 0 && (N & (N - 1)) == 0, "line must fit a wavefront, N power of two");
+    static_assert(N % EPT == 0 && TPF <= 64 && 64 % TPF == 0, "a line's threads must divide a wavefront");
     constexpr int SY = SYNC_WAVE;
     constexpr int MP = fft_lds_len(N, EPT, true);
     __shared__ C lds[WPB * LPW * MP];
@@ -980,9 +981,6 @@ __global__ void __launch_bounds__(64 * WPB, OCC) passz_full_kernel(PZArgs a) {
 
     long long g = (long long)blockIdx.x * WPB + wv;
     C cu[EPT], cw[EPT];  // raw coefficients of the next pair
-    R pzh[EPT];          // P_z of a line group waiting for its partner
-    C* pz_out = nullptr;
-    bool pz_held = false, pz_live = false;
     if (PF && g < ngroups) {
         long long gl = g * LPW + sub;
         if (gl >= total) gl = total - 1;
@@ -994,7 +992,10 @@ __global__ void __launch_bounds__(64 * WPB, OCC) passz_full_kernel(PZArgs a) {
         long long gl = g * LPW + sub;
         const bool live = gl < total;
         if (!live) gl = total - 1;  // idle lanes recompute the last line but never store
+        // boxes of the group's first and last line (uniform); they differ when N^2 is not a
+        // multiple of the lines per wavefront (N = 12: 144 lines, 32 per wavefront)
         const int box = (int)((g * LPW) / nlines);
+        const int box_last = (int)(((g + 1) * LPW - 1 < total ? (g + 1) * LPW - 1 : total - 1) / nlines);
         if (box != mx_box) {
             flush_max();
             mx_box = box;
@@ -1036,37 +1037,39 @@ __global__ void __launch_bounds__(64 * WPB, OCC) passz_full_kernel(PZArgs a) {
             pq[m] = cmk<C>(uy * wzv - uz * wy, uz * wx - ux * wzv);  // P_x + i P_y
             pz[m] = ux * wy - uy * wx;
         }
-        if (a.track_max && live) {
+        if (a.track_max) {
+            R ml = R(0);
 #pragma unroll
             for (int m = 0; m < EPT; ++m) {
                 const R v = fabs(a3[m].x) + fabs(a1[m].x) + fabs(a2[m].x);
-                mx = (v >= mx || v != v) ? v : mx;
+                ml = (v >= ml || v != v) ? v : ml;
+            }
+            if (!live) ml = R(0);
+            if (box == box_last) {
+                mx = (ml >= mx || ml != ml) ? ml : mx;
+            } else {
+                // the group straddles boxes box and box + 1: reduce each box's part now
+                const bool mine = (int)(gl / nlines) == box;
+                double va = mine ? (double)ml * a.scale : 0.0, vb = mine ? 0.0 : (double)ml * a.scale;
+                for (int off = 32; off > 0; off >>= 1) {
+                    va = nanmax(va, __shfl_xor(va, off));
+                    vb = nanmax(vb, __shfl_xor(vb, off));
+                }
+                if (lane == 0) {
+                    atomicMax(&a.sc[box].maxbits, (unsigned long long)__double_as_longlong(va));
+                    atomicMax(&a.sc[box_last].maxbits, (unsigned long long)__double_as_longlong(vb));
+                }
             }
         }
         zf_forward_store<T, N, EPT, SY, TAB, true>(pq, out + 0 * fstride, out + 1 * fstride, t, lane, lds, base, wz, live,
                                                    live);
-        // P_z of this line group goes into the imaginary part of the previous group's P_z
-        // transform (one forward transform for two lines); an unpaired one is transformed alone
-        if (!pz_held) {
-#pragma unroll
-            for (int m = 0; m < EPT; ++m) pzh[m] = pz[m];
-            pz_out = out + 2 * fstride;
-            pz_live = live;
-            pz_held = true;
-        } else {
-            C zz[EPT];
-#pragma unroll
-            for (int m = 0; m < EPT; ++m) zz[m] = cmk<C>(pzh[m], pz[m]);
-            zf_forward_store<T, N, EPT, SY, TAB, true>(zz, pz_out, out + 2 * fstride, t, lane, lds, base, wz, pz_live,
-                                                       live);
-            pz_held = false;
-        }
-    }
-    if (pz_held) {
+        // P_z alone (real input).  Pairing it with another line's P_z would save half a transform
+        // but make a line's rounding depend on which line it was paired with, i.e. on the grid, and
+        // a box of a batch would no longer evolve bitwise as it does alone.
         C zz[EPT];
 #pragma unroll
-        for (int m = 0; m < EPT; ++m) zz[m] = cmk<C>(pzh[m], R(0));
-        zf_forward_store<T, N, EPT, SY, TAB, false>(zz, pz_out, nullptr, t, lane, lds, base, wz, pz_live, false);
+        for (int m = 0; m < EPT; ++m) zz[m] = cmk<C>(pz[m], R(0));
+        zf_forward_store<T, N, EPT, SY, TAB, false>(zz, out + 2 * fstride, nullptr, t, lane, lds, base, wz, live, false);
     }
     flush_max();
 }

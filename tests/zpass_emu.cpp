@@ -59,8 +59,9 @@ int run(int ver, int nbox, unsigned blocks, const void* in, void* out, double sc
     a.kmax = Ext<N>::kmax;
     a.NZP = Ext<N>::NZP;
     a.track_max = 1;
-    constexpr bool half_ok = (N / 2) % EPTZ == 0 && (N / 2) / EPTZ <= 64 && 64 % ((N / 2) / EPTZ) == 0;
-    constexpr bool full_ok = N % EPTZ == 0 && N / EPTZ <= 64 && 64 % (N / EPTZ) == 0;
+    constexpr bool half_ok = (N / 2) % EPTZ == 0 && (N / 2) / EPTZ <= 64 && 64 % ((N / 2) / EPTZ) == 0 &&
+                             ((N / 2) & (N / 2 - 1)) == 0 && schedule_ok(N / 2, EPTZ);
+    constexpr bool full_ok = N % EPTZ == 0 && N / EPTZ <= 64 && 64 % (N / EPTZ) == 0 && schedule_ok(N, EPTZ);
     if (ver <= 3) {
         if constexpr (half_ok) {
             if (ver == 1) launch(passz_rhs_kernel<T, N, EPTZ, WPB, 1>, a, blocks, 64 * WPB);
@@ -87,9 +88,12 @@ template <typename T, int N> int by_ept(int ver, int eptz, int nbox, unsigned bl
                                         double scale, double* mx) {
     switch (eptz) {
         case 2: return run<T, N, 2>(ver, nbox, blocks, in, out, scale, mx);
-        case 4: return run<T, N, 4>(ver, nbox, blocks, in, out, scale, mx);
-        case 8: return run<T, N, 8>(ver, nbox, blocks, in, out, scale, mx);
-        case 16: if constexpr (N >= 16) return run<T, N, 16>(ver, nbox, blocks, in, out, scale, mx); break;
+        case 4: if constexpr (N % 4 == 0) return run<T, N, 4>(ver, nbox, blocks, in, out, scale, mx); break;
+        case 8: if constexpr (N % 8 == 0) return run<T, N, 8>(ver, nbox, blocks, in, out, scale, mx); break;
+        case 16: if constexpr (N >= 16 && N % 16 == 0) return run<T, N, 16>(ver, nbox, blocks, in, out, scale, mx); break;
+        case 3: if constexpr (N % 3 == 0) return run<T, N, 3>(ver, nbox, blocks, in, out, scale, mx); break;
+        case 6: if constexpr (N % 6 == 0) return run<T, N, 6>(ver, nbox, blocks, in, out, scale, mx); break;
+        case 12: if constexpr (N % 12 == 0) return run<T, N, 12>(ver, nbox, blocks, in, out, scale, mx); break;
         default: break;
     }
     return -1;
@@ -114,6 +118,10 @@ int emu_passz_rhs(int N, int fp32, int eptz, int ver, int nbox, int blocks, cons
             case 32: return by_ept<T, 32>(ver, eptz, nbox, blocks, in, out, scale, max_per_box);
             case 64: return by_ept<T, 64>(ver, eptz, nbox, blocks, in, out, scale, max_per_box);
             case 128: return by_ept<T, 128>(ver, eptz, nbox, blocks, in, out, scale, max_per_box);
+            case 12: return by_ept<T, 12>(ver, eptz, nbox, blocks, in, out, scale, max_per_box);
+            case 24: return by_ept<T, 24>(ver, eptz, nbox, blocks, in, out, scale, max_per_box);
+            case 48: return by_ept<T, 48>(ver, eptz, nbox, blocks, in, out, scale, max_per_box);
+            case 96: return by_ept<T, 96>(ver, eptz, nbox, blocks, in, out, scale, max_per_box);
             default: return -1;
         }
     };
