@@ -61,7 +61,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-KERNEL_OF_PASS = [r"pass1_kernel<{T}, {N},", r"passy_kernel<{T}, {N}, .*, 1>$", r"passz_rhs_kernel<{T}, {N},",
+KERNEL_OF_PASS = [r"pass1_kernel<{T}, {N},", r"passy_kernel<{T}, {N}, .*, 1>$", r"passz_(rhs|full)_kernel<{T}, {N},",
                   r"passy_kernel<{T}, {N}, .*, -1>$", r"pass5_kernel<{T}, {N},"]
 
 
