@@ -181,7 +181,8 @@ def make_params(N: int, nu: float = 1.0, dt: float = 1e-3, visc_pow: int = 1, sc
 
 
 def schedule_count(N: int) -> int:
-    """Number of FFT launch schedules the library has for size N (0 if N is unsupported)."""
+    """Number of launch schedules (FFT schedule x persistent z-pass grid) the library has for size N
+    (0 if N is unsupported)."""
     return load_library().nsg_schedule_count(int(N))
 
 
@@ -335,8 +336,8 @@ class Solver:
         return out
 
     def autotune(self, iters: int = 3, use_cache: bool = True):
-        """Time every FFT schedule of this size on the current state (restored afterwards) and keep
-        the fastest; returns (chosen schedule, ms per step of each schedule or None if cached)."""
+        """Time every launch schedule of this size on the current state (restored afterwards) and
+        keep the fastest; returns (chosen schedule, ms per step of each schedule or None if cached)."""
         n = schedule_count(self.N)
         ch = ctypes.c_int()
         ms = (ctypes.c_double * max(n, 1))(*([-1.0] * max(n, 1)))

@@ -28,10 +28,14 @@
 using namespace nsg;
 
 // ------------------------------------------------------------------------------------------
+// Launch schedules = FFT schedule x persistent z-pass grid.
 // FFT schedules.  EX/EY/EZ: elements per thread of the x, y (length N) and z (length N/2)
 // transforms, which fixes the radix sequence (nsg_fft.hpp); WX/WY: kz columns per block of the x
-// and y passes.  Schedule 0 is the default; the ensemble sizes 32..96 have two alternatives that
-// nsg_autotune times: 1 = shorter radices (more threads per transform), 2 = wider column tiles.
+// and y passes.  FFT schedule 0 is the default; the ensemble sizes 32..96 have two alternatives:
+// 1 = shorter radices (more threads per transform), 2 = wider column tiles.
+// z grids: the persistent z pass runs kZGrid[g] blocks per CU (capped by its work); the grid
+// changes no result bit (every line is transformed the same way whichever wavefront takes it, and
+// the per-box max is exact).  Schedule s = g * (FFT schedules of N) + f; nsg_autotune times all.
 // ------------------------------------------------------------------------------------------
 struct SchedV {
     int ex64, ex32, ey64, ey32, ez64, ez32, wx64, wx32, wy64, wy32;
@@ -52,7 +56,9 @@ constexpr SchedV sched0(int N) {
            : N == 512 ? SchedV{8, 8, 8, 16, 4, 8, 8, 16, 8, 16}
                       : SchedV{16, 16, 16, 16, 8, 8, 8, 16, 8, 16};
 }
-constexpr int nsched(int N) { return (N == 32 || N == 48 || N == 64 || N == 96) ? 3 : 1; }
+constexpr int nsched(int N) { return (N == 32 || N == 48 || N == 64 || N == 96) ? 3 : 1; }  // FFT schedules
+constexpr int kZGrid[] = {8, 4, 12};
+constexpr int kNZGrid = (int)(sizeof(kZGrid) / sizeof(kZGrid[0]));
 constexpr SchedV sched_of(int N, int S) {
     SchedV v = sched0(N);
     if (S == 1) {
@@ -103,6 +109,12 @@ template <typename F> static int dispatch_NS(int N, int S, F&& f) {
         }
         return f(n, std::integral_constant<int, 0>{});
     });
+}
+
+// FFT schedule of a context's launch schedule
+static int fft_sched(const nsg_ctx* c) {
+    const int nf = dispatch_N(c->N, [](auto nn) { return nsched(decltype(nn)::value); });
+    return nf > 0 ? c->sched % nf : 0;
 }
 
 // cos/sin(2 pi e / n) with octant reduction (exact at multiples of n/8)
@@ -266,6 +278,11 @@ template <typename T, int N> static constexpr int zfull_occ() {
     else return (N == 48 || N == 96) ? 2 : 3;
 }
 
+// P_z of two lines per forward transform (passz_full_kernel's ZPAIR): measured at 512^3 FP32
+// 0.742 -> 0.724 ms, FP64 1.333 -> 1.305 ms per launch; at 256^3 FP32 0.170 -> 0.178 ms (the
+// paired build spills there), so only 512^3 pairs
+template <int N> static constexpr bool zfull_pair() { return N == 512; }
+
 template <typename T, int N, int S>
 static int launch_passz_rhs(nsg_ctx* c, int b0, int nb, double scale, int track_max) {
     constexpr int EF = zfull_ept<T, N>();
@@ -279,12 +296,18 @@ static int launch_passz_rhs(nsg_ctx* c, int b0, int nb, double scale, int track_
     a.twM = twM_of<T>(c); a.twN = twN_of<T>(c); a.sc = c->sc + b0;
     a.scale = scale; a.kmax = c->kmax; a.NZP = c->NZP; a.track_max = track_max;
     const long long groups = ((long long)N * N * nb + LPW - 1) / LPW;
-    long long blocks = (groups + WPB - 1) / WPB;
-    const long long cap = (long long)c->ncu * c->zblocks_per_cu;
+    // the full form takes two line groups per wavefront iteration when they pair up
+    long long iters = groups;
+    if constexpr (EF > 0) {
+        if constexpr (zfull_pair<N>() && zfull_pairs<N, EF>()) iters = groups / 2;
+    }
+    long long blocks = (iters + WPB - 1) / WPB;
+    const int nf = nsched(N), zg = kZGrid[(c->sched / nf) % kNZGrid];
+    const long long cap = (long long)c->ncu * (c->zblocks_env > 0 ? c->zblocks_env : zg);
     if (blocks > cap) blocks = cap;
     prof_begin(c);
     if constexpr (EF > 0) {
-        hipLaunchKernelGGL((passz_full_kernel<T, N, EF, WPB, zfull_occ<T, N>(), true, false>), dim3((unsigned)blocks),
+        hipLaunchKernelGGL((passz_full_kernel<T, N, EF, WPB, zfull_occ<T, N>(), true, zfull_pair<N>()>), dim3((unsigned)blocks),
                            dim3(64 * WPB), 0, c->s, a);
     } else {
         // Fields of at most 16 VGPRs per thread: compiled for 2 waves per SIMD.  The kernel's natural
@@ -407,7 +430,7 @@ template <typename T, int N, int S> static int run_step(nsg_ctx* c, int b0, int 
 
 // enqueue n steps of boxes [b0, b0+nb) on the batch's stream
 static int enqueue_steps(nsg_ctx* c, long n, int b0, int nb) {
-    return dispatch_NS(c->N, c->sched, [&](auto nn, auto ss) {
+    return dispatch_NS(c->N, fft_sched(c), [&](auto nn, auto ss) {
         constexpr int NN = decltype(nn)::value, SS = decltype(ss)::value;
         for (long i = 0; i < n; ++i) {
             const int rc = c->fp32 ? run_step<float, NN, SS>(c, b0, nb) : run_step<double, NN, SS>(c, b0, nb);
@@ -515,7 +538,7 @@ static int create_batch(const nsg_params* p, int nbox, nsg_ctx** out) {
         HIPCHK(c, hipGetDeviceProperties(&prop, p[0].device));
         c->ncu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
         const char* zb = std::getenv("NSG_ZBLOCKS_PER_CU");
-        if (zb && std::atoi(zb) > 0) c->zblocks_per_cu = std::atoi(zb);
+        if (zb && std::atoi(zb) > 0) c->zblocks_env = std::atoi(zb);
     }
     HIPCHK(c, hipEventCreate(&c->ev[0]));
     HIPCHK(c, hipEventCreate(&c->ev[1]));
@@ -804,7 +827,7 @@ int nsg_rhs(nsg_ctx* c, const double* u_hat, double* out) {
     HIPCHK(c, hipSetDevice(c->p.device));
     int rc = set_state_box(c, 0, u_hat, c->aux);
     if (rc) return rc;
-    rc = dispatch_NS(c->N, c->sched, [&](auto nn, auto ss) {
+    rc = dispatch_NS(c->N, fft_sched(c), [&](auto nn, auto ss) {
         constexpr int NN = decltype(nn)::value, SS = decltype(ss)::value;
         return c->fp32 ? run_stage<float, NN, SS>(c, c->aux, 0, 1, 1, false, false)
                        : run_stage<double, NN, SS>(c, c->aux, 0, 1, 1, false, false);
@@ -970,7 +993,7 @@ int nsg_profile_passes(nsg_ctx* c, int iters, double* ms, double* bytes) {
 // ------------------------------------------------------------------------------------------
 int nsg_schedule_count(int N) {
     const int n = dispatch_N(N, [](auto nn) { return nsched(decltype(nn)::value); });
-    return n > 0 ? n : 0;
+    return n > 0 ? n * kNZGrid : 0;
 }
 
 int nsg_set_schedule(nsg_ctx* c, int sched) {

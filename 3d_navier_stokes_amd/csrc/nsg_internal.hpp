@@ -19,7 +19,8 @@ struct nsg_ctx {
     int N = 0, kmax = 0, NK = 0, NZP = 0, nspect = 0, nbox = 1;
     int sched = 0;
     bool fp32 = false, any_cfl = false;
-    int ncu = 256, zblocks_per_cu = 8;  // persistent z-pass grid = ncu * zblocks_per_cu blocks
+    int ncu = 256;
+    int zblocks_env = 0;  // NSG_ZBLOCKS_PER_CU: overrides the schedule's persistent z-pass grid
     hipStream_t s = nullptr;
     // per-box element strides of the state/K arrays and of A1..A4
     long long sbox = 0, a1box = 0, a2box = 0, a3box = 0, a4box = 0;

@@ -60,6 +60,19 @@ template <int N> __device__ __forceinline__ long long xidx(int x, int iy, int kz
     return (long long)x * xplane(NK, NZP) + (long long)iy * NZP + kz;
 }
 
+// Element i of an array whose base is wave-uniform, addressed by a 32-bit byte offset when O32
+// (the array must then be shorter than 4 GiB from base): the compiler keeps the base in SGPRs and
+// one VGPR per address (the saddr form of the global memory instructions) instead of a 64-bit
+// address pair per access, which is what bounds the register budget of the x passes.
+template <bool O32, typename E> __device__ __forceinline__ E& at(E* base, long long i) {
+    if constexpr (O32) {
+        using B = typename std::conditional<std::is_const<E>::value, const char, char>::type;
+        return *(E*)((B*)base + (unsigned)((unsigned)i * (unsigned)sizeof(E)));
+    } else {
+        return base[i];
+    }
+}
+
 // ------------------------------------------------------------------------------------------
 // per-mode helpers
 // ------------------------------------------------------------------------------------------
@@ -936,13 +949,20 @@ __device__ __forceinline__ void zf_forward_store(cpx_t<T> (&z)[EPT], cpx_t<T>* o
     }
 }
 
-// PF: the first pair of the next line group is fetched during the current group's third inverse
-// transform (otherwise at the top of each iteration, which frees its registers)
-template <typename T, int N, int EPT, int WPB, int OCC, bool TAB, bool PF = true>
+// ZPAIR: when the line groups of a box come in pairs ((N*N) % (2*LPW) == 0) a wavefront takes two
+// consecutive groups per iteration, and the real P_z of line l (first group) and of line l + LPW
+// (second group) share one forward transform, P_z(l) + i P_z(l + LPW), separated like P_x + i P_y.
+// Four complex transforms per line instead of five.  The partner of a line is fixed by the line
+// alone (box offsets are multiples of N*N, hence of 2*LPW), so a box's bits do not depend on the
+// grid or on the other boxes of a batch.
+template <int N, int EPT> constexpr bool zfull_pairs() { return (N * N) % (2 * (64 / (N / EPT))) == 0; }
+template <typename T, int N, int EPT, int WPB, int OCC, bool TAB, bool ZPAIR = true>
 __global__ void __launch_bounds__(64 * WPB, OCC) passz_full_kernel(PZArgs a) {
     using C = cpx_t<T>;
     constexpr int TPF = N / EPT, LPW = 64 / TPF;
     static_assert(N % EPT == 0 && TPF <= 64 && 64 % TPF == 0, "a line's threads must divide a wavefront");
+    constexpr bool PZ = ZPAIR && zfull_pairs<N, EPT>();
+    constexpr int GPI = PZ ? 2 : 1;  // line groups per iteration
     constexpr int SY = SYNC_WAVE;
     constexpr int MP = fft_lds_len(N, EPT, true);
     __shared__ C lds[WPB * LPW * MP];
@@ -959,6 +979,7 @@ __global__ void __launch_bounds__(64 * WPB, OCC) passz_full_kernel(PZArgs a) {
     const long long nlines = (long long)N * N;
     const long long total = nlines * a.nbox;
     const long long ngroups = (total + LPW - 1) / LPW;
+    const long long niter = PZ ? ngroups / 2 : ngroups;  // PZ: total is a multiple of 2 LPW
     const long long fstride = nlines * NZP;
     const C* in0 = (const C*)a.in;
     C* out0 = (C*)a.out;
@@ -979,97 +1000,97 @@ __global__ void __launch_bounds__(64 * WPB, OCC) passz_full_kernel(PZArgs a) {
         mx = R(0);
     };
 
-    long long g = (long long)blockIdx.x * WPB + wv;
-    C cu[EPT], cw[EPT];  // raw coefficients of the next pair
-    if (PF && g < ngroups) {
-        long long gl = g * LPW + sub;
-        if (gl >= total) gl = total - 1;
-        const C* in = in0 + line_off(gl, a.ibox);
-        zf_load<T, N, EPT>(cu, in + 1 * fstride, t);  // u_y
-        zf_load<T, N, EPT>(cw, in + 5 * fstride, t);  // w_z
-    }
-    for (; g < ngroups; g += (long long)gridDim.x * WPB) {
-        long long gl = g * LPW + sub;
-        const bool live = gl < total;
-        if (!live) gl = total - 1;  // idle lanes recompute the last line but never store
-        // boxes of the group's first and last line (uniform); they differ when N^2 is not a
-        // multiple of the lines per wavefront (N = 12: 144 lines, 32 per wavefront)
-        const int box = (int)((g * LPW) / nlines);
-        const int box_last = (int)(((g + 1) * LPW - 1 < total ? (g + 1) * LPW - 1 : total - 1) / nlines);
-        if (box != mx_box) {
-            flush_max();
-            mx_box = box;
-        }
-        const C* in = in0 + line_off(gl, a.ibox);
-        C* out = out0 + line_off(gl, a.obox);
-        if constexpr (!PF) {
-            zf_load<T, N, EPT>(cu, in + 1 * fstride, t);
-            zf_load<T, N, EPT>(cw, in + 5 * fstride, t);
-        }
-        C a1[EPT], a2[EPT], a3[EPT], nu[EPT], nw[EPT];
-        // (u_y, w_z): P_x = u_y w_z
-        zf_load<T, N, EPT>(nu, in + 2 * fstride, t);  // u_z
-        zf_load<T, N, EPT>(nw, in + 4 * fstride, t);  // w_y
-        zf_pack<T, N, EPT>(a1, cu, cw, t);
-        zfft<N, EPT, +1, SY, TAB>(a1, t, lds, base, wz);
-        // (u_z, w_y): P_x -= u_z w_y
-        zf_load<T, N, EPT>(cu, in + 0 * fstride, t);  // u_x
-        zf_load<T, N, EPT>(cw, in + 3 * fstride, t);  // w_x
-        zf_pack<T, N, EPT>(a2, nu, nw, t);
-        zfft<N, EPT, +1, SY, TAB>(a2, t, lds, base, wz);
-        // (u_x, w_x); meanwhile (u_y, w_z) of the next line group is fetched
-        const long long gn = g + (long long)gridDim.x * WPB;
-        zf_pack<T, N, EPT>(a3, cu, cw, t);
-        if (PF && gn < ngroups) {
-            long long ln = gn * LPW + sub;
-            if (ln >= total) ln = total - 1;
-            const C* inn = in0 + line_off(ln, a.ibox);
-            zf_load<T, N, EPT>(cu, inn + 1 * fstride, t);
-            zf_load<T, N, EPT>(cw, inn + 5 * fstride, t);
-        }
-        zfft<N, EPT, +1, SY, TAB>(a3, t, lds, base, wz);
-        // a1 = (u_y, w_z), a2 = (u_z, w_y), a3 = (u_x, w_x) at the grid points t + TPF m
-        C pq[EPT];
-        R pz[EPT];
-#pragma unroll
-        for (int m = 0; m < EPT; ++m) {
-            const auto uy = a1[m].x, wzv = a1[m].y, uz = a2[m].x, wy = a2[m].y, ux = a3[m].x, wx = a3[m].y;
-            pq[m] = cmk<C>(uy * wzv - uz * wy, uz * wx - ux * wzv);  // P_x + i P_y
-            pz[m] = ux * wy - uy * wx;
-        }
-        if (a.track_max) {
-            R ml = R(0);
+    for (long long h = (long long)blockIdx.x * WPB + wv; h < niter; h += (long long)gridDim.x * WPB) {
+        R pz0[PZ ? EPT : 1];  // P_z of the iteration's first line (PZ)
+        C* out_first = nullptr;
+#pragma unroll 1
+        for (int j = 0; j < GPI; ++j) {
+            const long long g = h * GPI + j;
+            long long gl = g * LPW + sub;
+            const bool live = gl < total;
+            if (!live) gl = total - 1;  // idle lanes recompute the last line but never store
+            // boxes of the group's first and last line (uniform); they differ when N^2 is not a
+            // multiple of the lines per wavefront (N = 12: 144 lines, 32 per wavefront)
+            const int box = (int)((g * LPW) / nlines);
+            const int box_last = (int)(((g + 1) * LPW - 1 < total ? (g + 1) * LPW - 1 : total - 1) / nlines);
+            if (box != mx_box) {
+                flush_max();
+                mx_box = box;
+            }
+            const C* in = in0 + line_off(gl, a.ibox);
+            C* out = out0 + line_off(gl, a.obox);
+            C a1[EPT], a2[EPT], a3[EPT], cu[EPT], cw[EPT];
+            // (u_y, w_z): P_x = u_y w_z
+            zf_load<T, N, EPT>(cu, in + 1 * fstride, t);  // u_y
+            zf_load<T, N, EPT>(cw, in + 5 * fstride, t);  // w_z
+            zf_pack<T, N, EPT>(a1, cu, cw, t);
+            zf_load<T, N, EPT>(cu, in + 2 * fstride, t);  // u_z
+            zf_load<T, N, EPT>(cw, in + 4 * fstride, t);  // w_y
+            zfft<N, EPT, +1, SY, TAB>(a1, t, lds, base, wz);
+            // (u_z, w_y): P_x -= u_z w_y
+            zf_pack<T, N, EPT>(a2, cu, cw, t);
+            zf_load<T, N, EPT>(cu, in + 0 * fstride, t);  // u_x
+            zf_load<T, N, EPT>(cw, in + 3 * fstride, t);  // w_x
+            zfft<N, EPT, +1, SY, TAB>(a2, t, lds, base, wz);
+            // (u_x, w_x)
+            zf_pack<T, N, EPT>(a3, cu, cw, t);
+            zfft<N, EPT, +1, SY, TAB>(a3, t, lds, base, wz);
+            // a1 = (u_y, w_z), a2 = (u_z, w_y), a3 = (u_x, w_x) at the grid points t + TPF m
+            C pq[EPT];
+            R pz[EPT];
 #pragma unroll
             for (int m = 0; m < EPT; ++m) {
-                const R v = fabs(a3[m].x) + fabs(a1[m].x) + fabs(a2[m].x);
-                ml = (v >= ml || v != v) ? v : ml;
+                const auto uy = a1[m].x, wzv = a1[m].y, uz = a2[m].x, wy = a2[m].y, ux = a3[m].x, wx = a3[m].y;
+                pq[m] = cmk<C>(uy * wzv - uz * wy, uz * wx - ux * wzv);  // P_x + i P_y
+                pz[m] = ux * wy - uy * wx;
             }
-            if (!live) ml = R(0);
-            if (box == box_last) {
-                mx = (ml >= mx || ml != ml) ? ml : mx;
+            if (a.track_max) {
+                R ml = R(0);
+#pragma unroll
+                for (int m = 0; m < EPT; ++m) {
+                    const R v = fabs(a3[m].x) + fabs(a1[m].x) + fabs(a2[m].x);
+                    ml = (v >= ml || v != v) ? v : ml;
+                }
+                if (!live) ml = R(0);
+                if (box == box_last) {
+                    mx = (ml >= mx || ml != ml) ? ml : mx;
+                } else {
+                    // the group straddles boxes box and box + 1: reduce each box's part now
+                    const bool mine = (int)(gl / nlines) == box;
+                    double va = mine ? (double)ml * a.scale : 0.0, vb = mine ? 0.0 : (double)ml * a.scale;
+                    for (int off = 32; off > 0; off >>= 1) {
+                        va = nanmax(va, __shfl_xor(va, off));
+                        vb = nanmax(vb, __shfl_xor(vb, off));
+                    }
+                    if (lane == 0) {
+                        atomicMax(&a.sc[box].maxbits, (unsigned long long)__double_as_longlong(va));
+                        atomicMax(&a.sc[box_last].maxbits, (unsigned long long)__double_as_longlong(vb));
+                    }
+                }
+            }
+            zf_forward_store<T, N, EPT, SY, TAB, true>(pq, out + 0 * fstride, out + 1 * fstride, t, lane, lds, base, wz,
+                                                       live, live);
+            C zz[EPT];
+            if constexpr (PZ) {
+                if (j == 0) {
+#pragma unroll
+                    for (int m = 0; m < EPT; ++m) pz0[m] = pz[m];
+                    out_first = out;
+                    continue;
+                }
+                // every line of a paired iteration is live (total is a multiple of 2 LPW)
+#pragma unroll
+                for (int m = 0; m < EPT; ++m) zz[m] = cmk<C>(pz0[m], pz[m]);
+                zf_forward_store<T, N, EPT, SY, TAB, true>(zz, out_first + 2 * fstride, out + 2 * fstride, t, lane, lds,
+                                                           base, wz, true, true);
             } else {
-                // the group straddles boxes box and box + 1: reduce each box's part now
-                const bool mine = (int)(gl / nlines) == box;
-                double va = mine ? (double)ml * a.scale : 0.0, vb = mine ? 0.0 : (double)ml * a.scale;
-                for (int off = 32; off > 0; off >>= 1) {
-                    va = nanmax(va, __shfl_xor(va, off));
-                    vb = nanmax(vb, __shfl_xor(vb, off));
-                }
-                if (lane == 0) {
-                    atomicMax(&a.sc[box].maxbits, (unsigned long long)__double_as_longlong(va));
-                    atomicMax(&a.sc[box_last].maxbits, (unsigned long long)__double_as_longlong(vb));
-                }
+                // P_z alone (real input)
+#pragma unroll
+                for (int m = 0; m < EPT; ++m) zz[m] = cmk<C>(pz[m], R(0));
+                zf_forward_store<T, N, EPT, SY, TAB, false>(zz, out + 2 * fstride, nullptr, t, lane, lds, base, wz, live,
+                                                            false);
             }
         }
-        zf_forward_store<T, N, EPT, SY, TAB, true>(pq, out + 0 * fstride, out + 1 * fstride, t, lane, lds, base, wz, live,
-                                                   live);
-        // P_z alone (real input).  Pairing it with another line's P_z would save half a transform
-        // but make a line's rounding depend on which line it was paired with, i.e. on the grid, and
-        // a box of a batch would no longer evolve bitwise as it does alone.
-        C zz[EPT];
-#pragma unroll
-        for (int m = 0; m < EPT; ++m) zz[m] = cmk<C>(pz[m], R(0));
-        zf_forward_store<T, N, EPT, SY, TAB, false>(zz, out + 2 * fstride, nullptr, t, lane, lds, base, wz, live, false);
     }
     flush_max();
 }
@@ -1109,11 +1130,19 @@ constexpr double DPE1 = DPB1 - 5179.0 / 57600.0, DPE3 = DPB3 - 7571.0 / 16695.0,
 // error tolerances (Solver/src/data_types.h:123-124)
 constexpr double DP_ATOL = 1e-7, DP_RTOL = 1e-7;
 
+template <typename T, int N, int EPT, int W> constexpr int p5_occ() {
+#ifdef NSG_P5_OCC
+    return (N / EPT * W == 512) ? NSG_P5_OCC : 1;
+#else
+    return 1;
+#endif
+}
 template <typename T, int N, int EPT, int W, int MODE, int FAM>
-__global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
+__global__ void __launch_bounds__(N / EPT * W, (p5_occ<T, N, EPT, W>())) pass5_kernel(P5Args a) {
     using C = cpx_t<T>;
     constexpr int TPF = N / EPT;
     constexpr bool PAIR = W * sizeof(C) < 128;
+    constexpr bool O32 = N <= 512;  // every component plane below 4 GiB
     __shared__ C lds[N * W];
     int tile, iy;
     if (!tile_of_block<PAIR>(a.NZP / W, a.NK, tile, iy)) return;
@@ -1140,7 +1169,7 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
 #pragma unroll
-        for (int m = 0; m < EPT; ++m) r[c][m] = in[c * iplane + xidx<N>(t + TPF * m, iy, kz, NK, NZP)];
+        for (int m = 0; m < EPT; ++m) r[c][m] = at<O32>(in + c * iplane, xidx<N>(t + TPF * m, iy, kz, NK, NZP));
         fft<N, EPT, -1>(r[c], t, lds, w, W, tw);
     }
 
@@ -1161,6 +1190,9 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
 
 #pragma unroll
     for (int m = 0; m < EPT; ++m) {
+#ifdef NSG_P5_SB
+        __builtin_amdgcn_sched_barrier(0);
+#endif
         const int p = t + TPF * m;
         const int kx = k_of_pos(p, N);
         if (kx < -kmax || kx > kmax || kz >= NZP) continue;
@@ -1173,7 +1205,7 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
             for (int c = 0; c < 3; ++c)
                 nl[c] = keep ? make_double2((double)r[c][m].x * a.norm, (double)r[c][m].y * a.norm) : make_double2(0.0, 0.0);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) uh[c * plane + idx] = nl[c];
+            for (int c = 0; c < 3; ++c) at<O32>(uh + c * plane, idx) = nl[c];
             continue;
         }
 #pragma unroll
@@ -1195,7 +1227,7 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
             if constexpr (FAM == 1) {
                 double2 us[3];
 #pragma unroll
-                for (int c = 0; c < 3; ++c) us[c] = uh[c * plane + idx];
+                for (int c = 0; c < 3; ++c) us[c] = at<O32>(uh + c * plane, idx);
                 add_slopes<T>(us, a.row, a.stage - 1, box * a.sbox, idx, plane, dt);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
@@ -1208,9 +1240,9 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
                 const C* kp = (a.stage == 2) ? K1 : (a.stage == 3) ? K2 : K3;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    double2 us = uh[c * plane + idx];
+                    double2 us = at<O32>(uh + c * plane, idx);
                     if (a.stage > 1) {
-                        double2 kv = to_d2<T>(kp[c * plane + idx]);
+                        double2 kv = to_d2<T>(at<O32>(kp + c * plane, idx));
                         us = make_double2(al * us.x + be * kv.x, al * us.y + be * kv.y);
                     }
                     nl[c].x += ffac * us.x;
@@ -1221,14 +1253,14 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
         if (extra && keep) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const double2 e = extra[c * plane + idx];
+                const double2 e = at<O32>(extra + c * plane, idx);
                 nl[c].x += e.x;
                 nl[c].y += e.y;
             }
         }
         if (a.stage < a.nstage) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) ko[c * plane + idx] = from_d2<T>(nl[c]);
+            for (int c = 0; c < 3; ++c) at<O32>(ko + c * plane, idx) = from_d2<T>(nl[c]);
         } else if constexpr (FAM == 1) {
             // u (2-D)/(2+D) + 2dt/(2+D) sum_i B_i K_i (Solver/src/solver.c:336-349); Dormand-Prince:
             // the candidate goes to unew and e = 2dt/(2+D) sum_i (B_i - Bs_i) K_i is measured
@@ -1240,13 +1272,13 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
             const bool dp = a.nstage == 7;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const double2 u0 = uh[c * plane + idx];
-                const double2 q1 = to_d2<T>(K1[c * plane + idx]);
-                const double2 q3 = to_d2<T>(K3[c * plane + idx]);
-                const double2 q4 = to_d2<T>(K4[c * plane + idx]);
-                const double2 q5 = to_d2<T>(K5[c * plane + idx]);
+                const double2 u0 = at<O32>(uh + c * plane, idx);
+                const double2 q1 = to_d2<T>(at<O32>(K1 + c * plane, idx));
+                const double2 q3 = to_d2<T>(at<O32>(K3 + c * plane, idx));
+                const double2 q4 = to_d2<T>(at<O32>(K4 + c * plane, idx));
+                const double2 q5 = to_d2<T>(at<O32>(K5 + c * plane, idx));
                 double2 q6 = nl[c];
-                if (dp) q6 = to_d2<T>(((const C*)a.row.k[5] + box * a.sbox)[c * plane + idx]);
+                if (dp) q6 = to_d2<T>(at<O32>((const C*)a.row.k[5] + box * a.sbox + c * plane, idx));
                 double2 un;
                 un.x = u0.x * g + h * (DPB1 * q1.x + DPB3 * q3.x + DPB4 * q4.x + DPB5 * q5.x + DPB6 * q6.x);
                 un.y = u0.y * g + h * (DPB1 * q1.y + DPB3 * q3.y + DPB4 * q4.y + DPB5 * q5.y + DPB6 * q6.y);
@@ -1259,9 +1291,9 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
                     const double a0 = sqrt(u0.x * u0.x + u0.y * u0.y), a1 = sqrt(un.x * un.x + un.y * un.y);
                     const double den = DP_ATOL + (a0 > a1 ? a0 : a1) * DP_RTOL;
                     if (keep) es[c] += (ex * ex + ey * ey) / (den * den);
-                    unew[c * plane + idx] = un;
+                    at<O32>(unew + c * plane, idx) = un;
                 } else {
-                    uh[c * plane + idx] = un;
+                    at<O32>(uh + c * plane, idx) = un;
                 }
             }
         } else {
@@ -1277,10 +1309,10 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
             }
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const double2 u0 = uh[c * plane + idx];
-                const double2 q1 = to_d2<T>(K1[c * plane + idx]);
-                const double2 q2 = to_d2<T>(K2[c * plane + idx]);
-                const double2 q3 = to_d2<T>(K3[c * plane + idx]);
+                const double2 u0 = at<O32>(uh + c * plane, idx);
+                const double2 q1 = to_d2<T>(at<O32>(K1 + c * plane, idx));
+                const double2 q2 = to_d2<T>(at<O32>(K2 + c * plane, idx));
+                const double2 q3 = to_d2<T>(at<O32>(K3 + c * plane, idx));
                 double2 un;
                 if (a.scheme == RK4CN) {
                     // u (2-D)/(2+D) + 2dt/(2+D) (K1/6 + K2/3 + K3/3 + K4/6), Solver/src/solver.c:601-603
@@ -1292,7 +1324,7 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
                     un.y = g * u0.y + dt / 6.0 * (g * q1.y + 2.0 * e2 * (q2.y + q3.y) + nl[c].y);
                 }
                 if (!keep) un = make_double2(0.0, 0.0);
-                uh[c * plane + idx] = un;
+                at<O32>(uh + c * plane, idx) = un;
             }
         }
     }

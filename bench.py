@@ -14,7 +14,7 @@ the same forced workload at 512^3.  `--cpu-plan` runs the whole CPU plan instead
 median of 3 steps after 1 warm-up, and the Taylor-Green configuration end to end).
 
 "ensemble_96_boxes" reports the batched-ensemble workload: 96 independent FP32-bulk boxes (32 at
-32^3, 32 at 48^3, 24 at 64^3, 8 at 96^3), 100 RK4 steps, autotuned FFT schedules, one HIP graph
+32^3, 32 at 48^3, 24 at 64^3, 8 at 96^3), 100 RK4 steps, autotuned launch schedules, one HIP graph
 replay per ensemble step, against the same steps launched box by box.
 
 "integrators" times the Dormand-Prince schemes on the same forced 512^3 workload: RK5-CN (six

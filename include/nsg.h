@@ -132,11 +132,14 @@ int nsg_profile_passes(nsg_ctx* ctx, int iters, double* ms, double* bytes);
 /* algorithmic HBM bytes of one full step (sum over its 5 x stages pass launches) */
 double nsg_step_bytes(const nsg_ctx* ctx);
 
-/* FFT launch schedules.  Every size has schedule 0; N = 32, 48, 64, 96 have alternatives
- * (1: shorter radices, more threads per transform; 2: wider column tiles).  Schedules differ only
- * in rounding.  nsg_autotune times `iters` steps of each on the context's own state, restores the
- * state and time afterwards, keeps the fastest and caches the choice per (N, precision) for the
- * process (use_cache = 0 re-measures).  ms_per_step, if not NULL, receives one entry per schedule.
+/* Launch schedules = FFT schedule f x persistent z-pass grid g, numbered s = g * F + f with F the
+ * FFT schedules of N.  Every size has FFT schedule 0; N = 32, 48, 64, 96 have two alternatives (1:
+ * shorter radices, more threads per transform; 2: wider column tiles), so F = 3 there and 1 elsewhere.
+ * The z grids are 8, 4 and 12 blocks per CU (g = 0, 1, 2).  FFT schedules differ only in rounding;
+ * the z grid changes no result bit.  nsg_schedule_count(N) = 3 F.  nsg_autotune times `iters` steps
+ * of each on the context's own state, restores the state and time afterwards, keeps the fastest and
+ * caches the choice per (N, precision, boxes) for the process (use_cache = 0 re-measures).
+ * ms_per_step, if not NULL, receives one entry per schedule.
  * No reference counterpart: the reference is planned by FFTW_MEASURE (Solver/src/solver.c:2034-2073). */
 int nsg_schedule_count(int N);
 int nsg_set_schedule(nsg_ctx* ctx, int sched);

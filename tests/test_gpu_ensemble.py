@@ -3,6 +3,7 @@
 Bars:
   * every FFT schedule of a size meets the single-box parity bars against the FP64 oracle
     (FP64 rel L2 <= 1e-12; FP32-bulk rhs rel L2 <= 2e-6);
+  * the persistent z-pass grid of a launch schedule changes no bit of a step;
   * a box stepped inside a batch / graph is BITWISE equal to the same box stepped alone under the
     same schedule (same kernels, per-box data, no cross-box reductions);
   * graph replays interleaved with eager work (measures, state reads, time reads) match the
@@ -26,7 +27,7 @@ def k4(oracle, N, seed, k0=2.0):
 @pytest.mark.parametrize("N", ALT_NS)
 @pytest.mark.parametrize("sched", [1, 2])
 def test_schedule_rhs_parity(nsg, oracle, N, sched):
-    assert nsg.schedule_count(N) == 3
+    assert nsg.schedule_count(N) == 9
     uh = k4(oracle, N, 77 + N)
     want = oracle.rhs(uh)
     with nsg.Solver(N) as s:
@@ -53,8 +54,28 @@ def test_schedule_steps_parity(nsg, oracle, N, sched):
 def test_schedule_out_of_range(nsg):
     with nsg.Solver(16) as s:
         with pytest.raises(nsg.NSGError):
-            s.set_schedule(1)
-    assert nsg.schedule_count(16) == 1 and nsg.schedule_count(17) == 0
+            s.set_schedule(3)
+    assert nsg.schedule_count(16) == 3 and nsg.schedule_count(17) == 0
+
+
+@pytest.mark.parametrize("N,prec", [(256, "fp32bulk"), (128, "fp64")])
+def test_z_grid_schedules_bitwise(nsg, oracle, N, prec):
+    """schedules 1 and 2 of a size without FFT alternatives differ from schedule 0 only in the
+    persistent z-pass grid (4 and 12 blocks per CU instead of 8), which must change no bit; the
+    launches differ here (N = 256 FP32: 8192 blocks of work on grids of 2048, 1024 and 3072 blocks;
+    N = 128 FP64: 2048 blocks of work, grids of 2048, 1024 and 2048)"""
+    assert nsg.schedule_count(N) == 3
+    uh = k4(oracle, N, 11)
+    got = []
+    for sched in range(3):
+        with nsg.Solver(N, nu=1e-3, dt=1e-3, cfl=0.5, precision=prec) as s:
+            s.set_schedule(sched)
+            s.set_state(uh)
+            s.step(2)
+            got.append((s.get_state(), s.time()))
+    for sched in (1, 2):
+        assert np.array_equal(got[sched][0], got[0][0]), f"schedule {sched}"
+        assert got[sched][1] == got[0][1]
 
 
 def mixed_boxes():
@@ -166,7 +187,7 @@ def test_autotune_restores_state(nsg, oracle):
         s.step(1)
         before, tb = s.get_state(), s.time()
         chosen, times = s.autotune(iters=2, use_cache=False)
-        assert 0 <= chosen < 3 and times is not None and len(times) == 3 and min(times) > 0
+        assert 0 <= chosen < 9 and times is not None and len(times) == 9 and min(times) > 0
         assert np.array_equal(s.get_state(), before) and s.time() == tb
         s.step(2)
         got = s.get_state()

@@ -62,15 +62,16 @@ CASES = [("fp64", 8, 4), ("fp64", 16, 4), ("fp64", 16, 8), ("fp64", 32, 8), ("fp
 
 
 # kernel versions: 1 passz_rhs_kernel; 2 passz_rhs2_kernel with base twiddles; 3 with the twiddle
-# table (elements per thread of the N/2-point transform); 4 passz_full_kernel with base twiddles, 5
-# with the twiddle table (elements per thread of the N-point transform: FULL lists the library's
-# instances for N <= 128, including the radix-3 sizes, and one line per wavefront, N/EPT = 64, the
-# geometry of the 512^3 instances)
+# table (elements per thread of the N/2-point transform); 4 passz_full_kernel with base twiddles and
+# P_z of two lines per transform, 5 with the twiddle table and P_z alone, 6 with the twiddle table
+# and P_z paired (elements per thread of the N-point transform: FULL lists the library's instances
+# for N <= 128, including the radix-3 sizes, and one line per wavefront, N/EPT = 64, the geometry
+# of the 512^3 instances; N = 12 has no pairs -- 144 lines, 32 per wavefront -- and runs unpaired)
 FULL = [("fp64", 8, 4), ("fp32", 8, 4), ("fp64", 12, 6), ("fp32", 12, 6), ("fp64", 16, 4), ("fp32", 16, 4),
         ("fp64", 24, 6), ("fp32", 24, 6), ("fp64", 32, 4), ("fp32", 32, 8), ("fp64", 48, 6), ("fp32", 48, 12),
         ("fp64", 64, 4), ("fp32", 64, 8), ("fp64", 96, 6), ("fp32", 96, 12), ("fp64", 128, 4), ("fp32", 128, 8),
         ("fp64", 128, 2), ("fp64", 64, 8), ("fp32", 16, 2)]
-RUNS = [(p, n, e, v) for (p, n, e) in CASES for v in (1, 2, 3)] + [(p, n, e, v) for (p, n, e) in FULL for v in (4, 5)
+RUNS = [(p, n, e, v) for (p, n, e) in CASES for v in (1, 2, 3)] + [(p, n, e, v) for (p, n, e) in FULL for v in (4, 5, 6)
                                                                    if not (n == 128 and e == 2 and v == 4)]
 
 

@@ -36,8 +36,9 @@ template <typename T> std::vector<cpx_t<T>> table(int L) {
 }
 
 // ver 1: passz_rhs_kernel; 2: passz_rhs2_kernel (base twiddles); 3: passz_rhs2_kernel (twiddle table);
-// 4: passz_full_kernel (base twiddles); 5: passz_full_kernel (twiddle table).  eptz is the elements per
-// thread of that kernel's transform (N/2 points for 1-3, N points for 4-5).
+// 4: passz_full_kernel (base twiddles, P_z of two lines per transform where the line groups pair up);
+// 5: passz_full_kernel (twiddle table, P_z alone); 6: passz_full_kernel (twiddle table, P_z paired).
+// eptz is the elements per thread of that kernel's transform (N/2 points for 1-3, N points for 4-6).
 template <typename T, int N, int EPTZ>
 int run(int ver, int nbox, unsigned blocks, const void* in, void* out, double scale, double* maxout) {
     constexpr int WPB = 4;
@@ -70,10 +71,11 @@ int run(int ver, int nbox, unsigned blocks, const void* in, void* out, double sc
         } else {
             return -1;
         }
-    } else if (ver <= 5) {
+    } else if (ver <= 6) {
         if constexpr (full_ok) {
-            if (ver == 4) launch(passz_full_kernel<T, N, EPTZ, WPB, 1, false>, a, blocks, 64 * WPB);
-            else launch(passz_full_kernel<T, N, EPTZ, WPB, 1, true>, a, blocks, 64 * WPB);
+            if (ver == 4) launch(passz_full_kernel<T, N, EPTZ, WPB, 1, false, true>, a, blocks, 64 * WPB);
+            else if (ver == 5) launch(passz_full_kernel<T, N, EPTZ, WPB, 1, true, false>, a, blocks, 64 * WPB);
+            else launch(passz_full_kernel<T, N, EPTZ, WPB, 1, true, true>, a, blocks, 64 * WPB);
         } else {
             return -1;
         }
