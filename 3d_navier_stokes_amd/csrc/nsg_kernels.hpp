@@ -1130,15 +1130,8 @@ constexpr double DPE1 = DPB1 - 5179.0 / 57600.0, DPE3 = DPB3 - 7571.0 / 16695.0,
 // error tolerances (Solver/src/data_types.h:123-124)
 constexpr double DP_ATOL = 1e-7, DP_RTOL = 1e-7;
 
-template <typename T, int N, int EPT, int W> constexpr int p5_occ() {
-#ifdef NSG_P5_OCC
-    return (N / EPT * W == 512) ? NSG_P5_OCC : 1;
-#else
-    return 1;
-#endif
-}
 template <typename T, int N, int EPT, int W, int MODE, int FAM>
-__global__ void __launch_bounds__(N / EPT * W, (p5_occ<T, N, EPT, W>())) pass5_kernel(P5Args a) {
+__global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
     using C = cpx_t<T>;
     constexpr int TPF = N / EPT;
     constexpr bool PAIR = W * sizeof(C) < 128;
@@ -1190,9 +1183,6 @@ __global__ void __launch_bounds__(N / EPT * W, (p5_occ<T, N, EPT, W>())) pass5_k
 
 #pragma unroll
     for (int m = 0; m < EPT; ++m) {
-#ifdef NSG_P5_SB
-        __builtin_amdgcn_sched_barrier(0);
-#endif
         const int p = t + TPF * m;
         const int kx = k_of_pos(p, N);
         if (kx < -kmax || kx > kmax || kz >= NZP) continue;
