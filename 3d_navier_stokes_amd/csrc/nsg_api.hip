@@ -81,6 +81,12 @@ template <typename T, int N, int S> static constexpr int WYof() { return is64<T>
 // 0.51 -> 0.48 ms per launch)
 template <typename T, int N, int S> static constexpr int E5of() { return (!is64<T>() && N == 512) ? 16 : EXof<T, N, S>(); }
 template <typename T, int N, int S> static constexpr int W5of() { return WXof<T, N, S>(); }
+// pass 5 with its field reads pipelined through LDS (pass5p_kernel) for the RK4-family stages at
+// 512^3: measured per launch FP32 0.334 -> 0.277 ms (stages 1-3) and 0.903 -> 0.865 ms (stage 4),
+// FP64 0.529 -> 0.494 and 1.296 -> 1.282 ms
+template <typename T, int N, int W> static constexpr bool p5_pipelined() {
+    return N == 512 && W * sizeof(cpx_t<T>) == 128;
+}
 
 template <typename F> static int dispatch_N(int N, F&& f) {
     switch (N) {
@@ -365,6 +371,18 @@ static int launch_pass5(nsg_ctx* c, double2* uh, int b0, int nb, int stage, void
     a.kmax = c->kmax; a.NK = c->NK; a.NZP = c->NZP;
     const dim3 grid = xgrid<T, W>(c->NZP / W, c->NK, nb);
     prof_begin(c);
+    if constexpr (MODE == EP_STAGE && p5_pipelined<T, N, W>()) {
+        if (!dp_family(c)) {
+            const int ntiles = c->NZP / W;
+            const long long nitems = (long long)ntiles * c->NK * nb;
+            const int blocks = (int)std::min<long long>(nitems, c->ncu);  // 2 N W elements of LDS: one block per CU
+            hipLaunchKernelGGL((pass5p_kernel<T, N, EPT, W>), dim3((unsigned)blocks), dim3(TPF * W), 0, c->s, a, ntiles,
+                               (int)nitems);
+            LAUNCHCHK(c);
+            prof_end(c, 4);
+            return NSG_OK;
+        }
+    }
     if constexpr (MODE == EP_STAGE) {
         if (dp_family(c))
             hipLaunchKernelGGL((pass5_kernel<T, N, EPT, W, MODE, 1>), grid, dim3(TPF * W), 0, c->s, a);

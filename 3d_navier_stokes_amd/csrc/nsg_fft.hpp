@@ -187,11 +187,16 @@ __device__ __forceinline__ void gather_pattern(C (&v)[EPT], int t, const C* lds,
 
 // LDS exchange barrier.  SYNC_BLOCK: the transform spans several wavefronts (s_barrier);
 // SYNC_WAVE: all TPF threads of a transform sit in one wavefront, which executes its LDS
-// instructions in order, so only a wavefront-scope fence is needed and waves run independently.
-enum SyncKind { SYNC_BLOCK = 0, SYNC_WAVE = 1 };
+// instructions in order, so only a wavefront-scope fence is needed and waves run independently;
+// SYNC_LDS: a block barrier that waits for the wavefront's own LDS accesses only, so global reads
+// and global->LDS copies into other LDS buffers stay in flight across the transform (the caller
+// keeps every LDS access of the transform away from those buffers).
+enum SyncKind { SYNC_BLOCK = 0, SYNC_WAVE = 1, SYNC_LDS = 2 };
 template <int SYNC> __device__ __forceinline__ void lds_sync() {
     if constexpr (SYNC == SYNC_BLOCK) {
         __syncthreads();
+    } else if constexpr (SYNC == SYNC_LDS) {
+        asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
     } else {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -220,7 +225,7 @@ __device__ __forceinline__ void fft(C (&v)[EPT], int t, C* lds, int base, int PS
     static_assert(schedule_ok(L, EPT), "unsupported FFT length / elements-per-thread");
     constexpr int TPF = L / EPT;
     constexpr int NS = num_stages(L, EPT);
-    static_assert(SYNC == SYNC_BLOCK || (TPF <= 64 && 64 % TPF == 0), "wave-synchronous transform must fit a wavefront");
+    static_assert(SYNC != SYNC_WAVE || (TPF <= 64 && 64 % TPF == 0), "wave-synchronous transform must fit a wavefront");
     fft_stages<L, EPT, TPF, 0, NS, DIR, SYNC, PAD>(v, t, lds, base, PS, tw);
 }
 

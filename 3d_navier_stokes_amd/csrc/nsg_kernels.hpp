@@ -1130,16 +1130,14 @@ constexpr double DPE1 = DPB1 - 5179.0 / 57600.0, DPE3 = DPB3 - 7571.0 / 16695.0,
 // error tolerances (Solver/src/data_types.h:123-124)
 constexpr double DP_ATOL = 1e-7, DP_RTOL = 1e-7;
 
-template <typename T, int N, int EPT, int W, int MODE, int FAM>
-__global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
+// Epilogue of one (box, row iy, column kz) after the forward x transforms: r holds the three fields
+// at the x positions t + TPF m; es accumulates this thread's Dormand-Prince error sums.
+template <typename T, int N, int EPT, int MODE, int FAM>
+__device__ __forceinline__ void p5_epilogue(const P5Args& a, const cpx_t<T> (&r)[3][EPT], int box, int iy, int kz, int t,
+                                            double (&es)[3]) {
     using C = cpx_t<T>;
     constexpr int TPF = N / EPT;
-    constexpr bool PAIR = W * sizeof(C) < 128;
     constexpr bool O32 = N <= 512;  // every component plane below 4 GiB
-    __shared__ C lds[N * W];
-    int tile, iy;
-    if (!tile_of_block<PAIR>(a.NZP / W, a.NK, tile, iy)) return;
-    const int box = blockIdx.z;
     const DevScalars& sc = a.sc[box];
     double2* __restrict__ uh = a.uh + box * a.sbox;
     double2* __restrict__ unew = a.unew ? a.unew + box * a.sbox : nullptr;  // null: update in place
@@ -1149,22 +1147,9 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
     C* ko = a.kout ? (C*)a.kout + box * a.sbox : nullptr;
     const double2* extra = a.extra ? a.extra + box * a.sbox : nullptr;
     const double* fslots = a.fslots + box * a.NK;
-    const int tid = threadIdx.x, w = tid % W, t = tid / W;
-    const int kz = tile * W + w;
     const int kmax = a.kmax, NK = a.NK, NZP = a.NZP;
     const int ky = k_of_cidx(iy, kmax, NK);
     const long long plane = (long long)NK * NK * NZP;
-    const long long iplane = (long long)N * xplane(NK, NZP);
-    const C* in = (const C*)a.in + box * a.ibox;
-    const C* tw = (const C*)a.twN;
-
-    C r[3][EPT];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-#pragma unroll
-        for (int m = 0; m < EPT; ++m) r[c][m] = at<O32>(in + c * iplane, xidx<N>(t + TPF * m, iy, kz, NK, NZP));
-        fft<N, EPT, -1>(r[c], t, lds, w, W, tw);
-    }
 
     double ffac = 0.0;
     if (MODE == EP_STAGE && a.force) {
@@ -1179,7 +1164,6 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
     }
     const double dt = sc.dt, nu = sc.nu;
     const long long kcut2 = (long long)kmax * kmax;
-    double es[3] = {0.0, 0.0, 0.0};  // Dormand-Prince error sums of this thread
 
 #pragma unroll
     for (int m = 0; m < EPT; ++m) {
@@ -1318,6 +1302,34 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
             }
         }
     }
+}
+
+template <typename T, int N, int EPT, int W, int MODE, int FAM>
+__global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
+    using C = cpx_t<T>;
+    constexpr int TPF = N / EPT;
+    constexpr bool PAIR = W * sizeof(C) < 128;
+    constexpr bool O32 = N <= 512;  // every component plane below 4 GiB
+    __shared__ C lds[N * W];
+    int tile, iy;
+    if (!tile_of_block<PAIR>(a.NZP / W, a.NK, tile, iy)) return;
+    const int box = blockIdx.z;
+    const int tid = threadIdx.x, w = tid % W, t = tid / W;
+    const int kz = tile * W + w;
+    const int NK = a.NK, NZP = a.NZP;
+    const long long iplane = (long long)N * xplane(NK, NZP);
+    const C* in = (const C*)a.in + box * a.ibox;
+    const C* tw = (const C*)a.twN;
+
+    C r[3][EPT];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) r[c][m] = at<O32>(in + c * iplane, xidx<N>(t + TPF * m, iy, kz, NK, NZP));
+        fft<N, EPT, -1>(r[c], t, lds, w, W, tw);
+    }
+    double es[3] = {0.0, 0.0, 0.0};  // Dormand-Prince error sums of this thread
+    p5_epilogue<T, N, EPT, MODE, FAM>(a, r, box, iy, kz, t, es);
     if (FAM == 1 && MODE == EP_STAGE && a.errpart) {
         // per-block error sums, reduced in a fixed order by dp_finalize_kernel
         __syncthreads();  // the transforms are done with lds
@@ -1328,6 +1340,73 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
             const double v = block_sum(es[c], red);
             if (tid == 0) dst[c] = v;
         }
+    }
+}
+
+
+// Pass 5 with its field reads pipelined (RK4 family, stage epilogue, one 128-byte segment per x
+// line: W * sizeof(C) == 128).  Persistent blocks walk the (box, row iy, column tile) items; each
+// field's N x W input goes from HBM straight into an LDS staging buffer through the global->LDS
+// DMA (64 lanes x 16 bytes per instruction, eight x lines) while the block transforms the field
+// before, so the reads of one field overlap the transform of the previous one.  (pass5_kernel
+// holds one such block per CU, which issues all its reads, then transforms with none in flight.)
+// grid: at most one block per CU (2 x N*W elements of LDS); items = boxes * NK * (NZP / W)
+template <typename T, int N, int EPT, int W>
+__global__ void __launch_bounds__(N / EPT * W) pass5p_kernel(P5Args a, int ntiles, int nitems) {
+    using C = cpx_t<T>;
+    constexpr int TPF = N / EPT, NT = TPF * W, NWAVE = NT / 64;
+    static_assert(W * sizeof(C) == 128, "one 128-byte segment per x line");
+    static_assert(NT % 64 == 0, "whole wavefronts");
+    constexpr int ROWS = 8;                // x lines per DMA instruction (64 lanes x 16 bytes)
+    static_assert(N % (ROWS * NWAVE) == 0, "whole DMA instructions per wavefront");
+    constexpr int INST = N / (ROWS * NWAVE);  // DMA instructions per wavefront and field
+    constexpr int EPL = 16 / (int)sizeof(C);  // complex elements per lane and instruction
+    __shared__ C lds[N * W];  // transform exchange
+    __shared__ C stg[N * W];  // staged input of one field, [x][W]
+    __shared__ C twl[N];      // twiddle table: transform reads from LDS keep the global counter free
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int NK = a.NK, NZP = a.NZP;
+    const long long xpl = xplane(NK, NZP), iplane = (long long)N * xpl;
+    for (int e = threadIdx.x; e < N; e += NT) twl[e] = ((const C*)a.twN)[e];
+    __syncthreads();
+    // DMA of field c of an item: lane l of instruction i of wavefront wv copies 16 bytes of x line
+    // (wv INST + i) ROWS + l / 8, columns tile W + (l % 8) EPL ..., to the same place in stg
+    auto issue = [&](int item, int c, int lane) {
+        const int tile = item % ntiles, rest = item / ntiles, iy = rest % NK, box = rest / NK;
+        const C* src = (const C*)a.in + box * a.ibox + c * iplane + (long long)iy * NZP + tile * W + (lane % 8) * EPL;
+#pragma unroll
+        for (int i = 0; i < INST; ++i) {
+            const int row0 = (wv * INST + i) * ROWS;
+            const C* g = src + (long long)(row0 + lane / 8) * xpl;
+            __builtin_amdgcn_global_load_lds((const void*)g, (__attribute__((address_space(3))) void*)(stg + row0 * W), 16, 0,
+                                             0);
+        }
+    };
+    int item = blockIdx.x;
+    if (item < nitems) issue(item, 0, threadIdx.x & 63);
+    for (; item < nitems; item += gridDim.x) {
+        // the thread's coordinates, opaque to the compiler on every item: what derives from them
+        // (transform and epilogue addresses) is recomputed per item instead of held in registers
+        // across the loop
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        const int w = tid % W, t = tid / W, lane = tid & 63;
+        const int tile = item % ntiles, rest = item / ntiles, iy = rest % NK, box = rest / NK;
+        const int kz = tile * W + w;
+        C r[3][EPT];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            // this wavefront's copies (and stores) are done, then every wavefront's
+            asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+#pragma unroll
+            for (int m = 0; m < EPT; ++m) r[c][m] = stg[(t + TPF * m) * W + w];
+            lds_sync<SYNC_LDS>();  // stg read by every thread before it is refilled
+            const int nitem = (c < 2) ? item : item + (int)gridDim.x;
+            if (nitem < nitems) issue(nitem, (c + 1) % 3, lane);
+            fft<N, EPT, -1, SYNC_LDS>(r[c], t, lds, w, W, twl);
+        }
+        double es[3] = {0.0, 0.0, 0.0};
+        p5_epilogue<T, N, EPT, EP_STAGE, 0>(a, r, box, iy, kz, t, es);
     }
 }
 

@@ -62,7 +62,7 @@ def log(msg):
 
 
 KERNEL_OF_PASS = [r"pass1_kernel<{T}, {N},", r"passy_kernel<{T}, {N}, .*, 1>$", r"passz_(rhs|full)_kernel<{T}, {N},",
-                  r"passy_kernel<{T}, {N}, .*, -1>$", r"pass5_kernel<{T}, {N},"]
+                  r"passy_kernel<{T}, {N}, .*, -1>$", r"pass5p?_kernel<{T}, {N},"]
 
 
 def measured_traffic(N, precision, pass_index):
