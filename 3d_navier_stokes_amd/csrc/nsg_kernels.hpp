@@ -225,34 +225,87 @@ __global__ void __launch_bounds__(N / EPT * W) pass1_kernel(P1Args a) {
 
     C u[3][EPT];  // stage input in the bulk precision (formed in FP64)
     double fpart = 0.0;
+    if constexpr (FAM == 1) {
 #pragma unroll
-    for (int m = 0; m < EPT; ++m) {
-        const int p = t + TPF * m;
-        const int kx = k_of_pos(p, N);
-        const bool ok = colok && kx >= -kmax && kx <= kmax;
-        const long long idx = sidx(cidx_of_k(kx, NK), iy, kz, NK, NZP);
-        const long long k2 = (long long)kx * kx + (long long)ky * ky + (long long)kz * kz;
-        double2 ud[3];
+        for (int m = 0; m < EPT; ++m) {
+            const int p = t + TPF * m;
+            const int kx = k_of_pos(p, N);
+            const bool ok = colok && kx >= -kmax && kx <= kmax;
+            const long long idx = sidx(cidx_of_k(kx, NK), iy, kz, NK, NZP);
+            const long long k2 = (long long)kx * kx + (long long)ky * ky + (long long)kz * kz;
+            double2 ud[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) ud[c] = ok ? uh[c * plane + idx] : make_double2(0.0, 0.0);
-        if constexpr (FAM == 1) {
+            for (int c = 0; c < 3; ++c) ud[c] = ok ? uh[c * plane + idx] : make_double2(0.0, 0.0);
             if (a.stage > 1 && ok) add_slopes<T>(ud, a.row, a.stage - 1, box * a.sbox, idx, plane, dt);
-        } else if (a.stage > 1 && ok) {
-            double al, be;
-            stage_in_coef(a.scheme, a.stage, visc_rate(nu, a.visc_pow, k2), dt, al, be);
+            if (a.force && ok && k2 > 0 && (double)k2 <= a.kf2) {
+                const double wgt = (kz == 0) ? 1.0 : 2.0;
+                fpart += wgt * (ud[0].x * ud[0].x + ud[0].y * ud[0].y + ud[1].x * ud[1].x + ud[1].y * ud[1].y +
+                                ud[2].x * ud[2].x + ud[2].y * ud[2].y);
+            }
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                double2 kv = to_d2<T>(kprev[c * plane + idx]);
-                ud[c] = make_double2(al * ud[c].x + be * kv.x, al * ud[c].y + be * kv.y);
+            for (int c = 0; c < 3; ++c) u[c][m] = from_d2<T>(ud[c]);
+        }
+    } else {
+        // RK4 family: every read of a round is issued before any is used (unconditional reads,
+        // positions outside the cut read element 0 of the plane and are zeroed afterwards), so they
+        // are in flight together instead of one position at a time; slots whose positions all lie
+        // outside the cut (|kx| > N/3 for every thread) are neither read nor transformed as data.
+        // FP32 takes two rounds of half the positions (register budget of a 1024-thread block).
+        constexpr int KM = Ext<N>::kmax;
+        constexpr int NH = (sizeof(T) == 4 && EPT % 2 == 0) ? 2 : 1, MH = EPT / NH;
+        const bool prev = a.stage > 1;
+        const bool rkcn = a.scheme == RK4CN;
+        const double be_cn = (a.stage == 4) ? dt : 0.5 * dt;
+#pragma unroll
+        for (int h = 0; h < NH; ++h) {
+            double2 ud[3][MH];
+            C kv[3][MH];
+#pragma unroll
+            for (int j = 0; j < MH; ++j) {
+                const int m = h * MH + j;
+                if (TPF * m > KM && TPF * m + TPF - 1 < N - KM) continue;  // folded once unrolled
+                const int kx = k_of_pos(t + TPF * m, N);
+                const bool ok = colok && kx >= -kmax && kx <= kmax;
+                const long long idx = ok ? sidx(cidx_of_k(kx, NK), iy, kz, NK, NZP) : 0;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) ud[c][j] = uh[c * plane + idx];
+                if (prev) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) kv[c][j] = kprev[c * plane + idx];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < MH; ++j) {
+                const int m = h * MH + j;
+                if (TPF * m > KM && TPF * m + TPF - 1 < N - KM) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) u[c][m] = cmk<C>(0, 0);
+                    continue;
+                }
+                const int kx = k_of_pos(t + TPF * m, N);
+                const bool ok = colok && kx >= -kmax && kx <= kmax;
+                const long long k2 = (long long)kx * kx + (long long)ky * ky + (long long)kz * kz;
+                double2 ud3[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) ud3[c] = ok ? ud[c][j] : make_double2(0.0, 0.0);
+                if (prev && ok) {
+                    double al = 1.0, be = be_cn;
+                    if (!rkcn) stage_in_coef(a.scheme, a.stage, visc_rate(nu, a.visc_pow, k2), dt, al, be);
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const double2 kd = to_d2<T>(kv[c][j]);
+                        ud3[c] = make_double2(al * ud3[c].x + be * kd.x, al * ud3[c].y + be * kd.y);
+                    }
+                }
+                if (a.force && ok && k2 > 0 && (double)k2 <= a.kf2) {
+                    const double wgt = (kz == 0) ? 1.0 : 2.0;
+                    fpart += wgt * (ud3[0].x * ud3[0].x + ud3[0].y * ud3[0].y + ud3[1].x * ud3[1].x +
+                                    ud3[1].y * ud3[1].y + ud3[2].x * ud3[2].x + ud3[2].y * ud3[2].y);
+                }
+#pragma unroll
+                for (int c = 0; c < 3; ++c) u[c][m] = from_d2<T>(ud3[c]);
             }
         }
-        if (a.force && ok && k2 > 0 && (double)k2 <= a.kf2) {
-            const double wgt = (kz == 0) ? 1.0 : 2.0;
-            fpart += wgt * (ud[0].x * ud[0].x + ud[0].y * ud[0].y + ud[1].x * ud[1].x + ud[1].y * ud[1].y +
-                            ud[2].x * ud[2].x + ud[2].y * ud[2].y);
-        }
-#pragma unroll
-        for (int c = 0; c < 3; ++c) u[c][m] = from_d2<T>(ud[c]);
     }
     if (a.force && tile == 0 && (double)ky * ky <= a.kf2) {
         double s = block_sum(fpart, red);
