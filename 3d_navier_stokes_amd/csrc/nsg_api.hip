@@ -285,9 +285,10 @@ template <typename T, int N> static constexpr int zfull_occ() {
 }
 
 // P_z of two lines per forward transform (passz_full_kernel's ZPAIR): measured at 512^3 FP32
-// 0.742 -> 0.724 ms, FP64 1.333 -> 1.305 ms per launch; at 256^3 FP32 0.170 -> 0.178 ms (the
-// paired build spills there), so only 512^3 pairs
-template <int N> static constexpr bool zfull_pair() { return N == 512; }
+// 0.742 -> 0.724 ms per launch; FP64 1.333 -> 1.305 ms alone but the same 1.39 ms inside the step,
+// with 10 % more HBM reads (the paired FP64 build spills 36 bytes per lane); at 256^3 FP32
+// 0.170 -> 0.178 ms (the paired build spills there).  So only FP32 at 512^3 pairs.
+template <typename T, int N> static constexpr bool zfull_pair() { return N == 512 && !is64<T>(); }
 
 template <typename T, int N, int S>
 static int launch_passz_rhs(nsg_ctx* c, int b0, int nb, double scale, int track_max) {
@@ -305,7 +306,7 @@ static int launch_passz_rhs(nsg_ctx* c, int b0, int nb, double scale, int track_
     // the full form takes two line groups per wavefront iteration when they pair up
     long long iters = groups;
     if constexpr (EF > 0) {
-        if constexpr (zfull_pair<N>() && zfull_pairs<N, EF>()) iters = groups / 2;
+        if constexpr (zfull_pair<T, N>() && zfull_pairs<N, EF>()) iters = groups / 2;
     }
     long long blocks = (iters + WPB - 1) / WPB;
     const int nf = nsched(N), zg = kZGrid[(c->sched / nf) % kNZGrid];
@@ -313,7 +314,7 @@ static int launch_passz_rhs(nsg_ctx* c, int b0, int nb, double scale, int track_
     if (blocks > cap) blocks = cap;
     prof_begin(c);
     if constexpr (EF > 0) {
-        hipLaunchKernelGGL((passz_full_kernel<T, N, EF, WPB, zfull_occ<T, N>(), true, zfull_pair<N>()>), dim3((unsigned)blocks),
+        hipLaunchKernelGGL((passz_full_kernel<T, N, EF, WPB, zfull_occ<T, N>(), true, zfull_pair<T, N>()>), dim3((unsigned)blocks),
                            dim3(64 * WPB), 0, c->s, a);
     } else {
         // Fields of at most 16 VGPRs per thread: compiled for 2 waves per SIMD.  The kernel's natural

@@ -196,6 +196,199 @@ __global__ void __launch_bounds__(CONV_THREADS) conv3_kernel(ConvArgs a) {
     }
 }
 
+// Forward convolution, persistent and pipelined (forward modes).  A 512-thread block walks the
+// (sample, tile) items of 4 x 4 x 16 output points; the 6 x 6 x 18 periodic halo of the next item
+// goes from HBM to LDS through the global->LDS DMA (64 lanes x 16 bytes per instruction) while the
+// MFMAs of the current item run, into the other of two halo buffers.  The layer's weights stay in
+// registers for the whole kernel: with 32 output channels the wavefronts split them (waves 0-3
+// hold channels 0-15, waves 4-7 channels 16-31, each over 4 z-rows), with 16 every wavefront holds
+// them all (over 2 z-rows), so LDS carries only the A operand and two wavefronts share a SIMD.
+// The DMA writes each halo point's CIN channels contiguously (no padding), so the 16-byte chunks
+// of a point are swizzled instead: chunk c of the point with halo z index hz sits in slot
+// c ^ ((hz >> SH) & (CH - 1)), which puts the 8 points that one LDS cycle serves in distinct
+// banks.  GEMM view per item as conv3_kernel's: M = 256 points (16 z-rows), N = COUT, K = 27 taps
+// x CIN in steps of 32.  The loop over items is unrolled by two so that the two halo buffers are
+// distinct LDS objects: the reads of one do not wait for the copies into the other.
+constexpr int PX = 4, PY = 4, PZ = 16;
+constexpr int PHY = PY + 2, PHZ = PZ + 2, PHPTS = (PX + 2) * PHY * PHZ, PPTS = PX * PY * PZ;
+constexpr int P_THREADS = 512;
+
+template <int CIN, int COUT, int EPI>
+__global__ void __launch_bounds__(P_THREADS, 2) conv3p_kernel(ConvArgs a, int ntile, int nitems) {
+    static_assert(EPI == EPI_GELU || EPI == EPI_GELU_PRE || EPI == EPI_LINEAR, "forward modes only");
+    constexpr int CH = CIN / 8;                  // 16-byte chunks per point
+    constexpr int SH = (CH == 4) ? 1 : 2;        // swizzle shift (CH = 2 or 4)
+    static_assert(CH == 2 || CH == 4, "CIN = 16 or 32");
+    constexpr int KT = 27 * CIN, KSTEPS = (KT + 31) / 32, NH = COUT / 16;
+    constexpr int NWAVE = P_THREADS / 64;
+    constexpr int HS = (NH >= 2) ? 2 : 1;        // channel groups of wavefronts
+    constexpr int NHW = NH / HS;                 // 16-channel blocks per wavefront
+    constexpr int SUB = PPTS / 16 / (NWAVE / HS);  // z-rows per wavefront
+    constexpr int OS = COUT + 8;                 // output tile point stride (halfwords)
+    constexpr int HB = PHPTS * CIN;              // halfwords per halo buffer
+    constexpr int NCHUNK = PHPTS * CH;
+    constexpr int ROUNDS = (NCHUNK + P_THREADS - 1) / P_THREADS;
+    static_assert(PPTS * OS <= HB, "output tile must fit in a halo buffer");
+    __shared__ __attribute__((aligned(16))) uint16_t halo0[HB];
+    __shared__ __attribute__((aligned(16))) uint16_t halo1[HB];
+    const int N = a.N;
+    const long long N3 = (long long)N * N * N;
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int m = lane & 15, g = lane >> 4;
+    const int wrow = wv % (NWAVE / HS), hgrp = wv / (NWAVE / HS);  // z-rows wrow SUB .., channels hgrp NHW 16 ..
+
+    // this thread's share of a halo copy, the same for every item: chunk f = (r NWAVE + wv) 64 + lane
+    // of point (hx, hy, hz) and source chunk src, packed as hx | hy << 4 | hz << 8 | src << 16
+    // (-1: past the halo)
+    int dmac[ROUNDS];
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+        const int f = (r * NWAVE + wv) * 64 + lane;
+        if (f < NCHUNK) {
+            const int q = f / CH, slot = f % CH;
+            const int hz = q % PHZ, hy = (q / PHZ) % PHY, hx = q / (PHZ * PHY);
+            dmac[r] = hx | (hy << 4) | (hz << 8) | ((slot ^ ((hz >> SH) & (CH - 1))) << 16);
+        } else {
+            dmac[r] = -1;
+        }
+    }
+    auto origin = [&](int item, int& b, int& x0, int& y0, int& z0) {
+        b = item / ntile;
+        int t = item % ntile;
+        const int ntz = N / PZ, nty = N / PY;
+        z0 = (t % ntz) * PZ;
+        t /= ntz;
+        y0 = (t % nty) * PY;
+        x0 = (t / nty) * PX;
+    };
+    auto issue = [&](int item, uint16_t* buf) {
+        int b, x0, y0, z0;
+        origin(item, b, x0, y0, z0);
+        const uint16_t* in = a.in + (long long)b * N3 * CIN;
+#pragma unroll
+        for (int r = 0; r < ROUNDS; ++r) {
+            const int f0 = (r * NWAVE + wv) * 64;  // uniform
+            const int d = dmac[r];
+            if (d >= 0) {
+                const int gx = wrap(x0 + (d & 15) - 1, N), gy = wrap(y0 + ((d >> 4) & 15) - 1, N);
+                const int gz = wrap(z0 + ((d >> 8) & 31) - 1, N);
+                const uint16_t* gp = in + (((long long)gx * N + gy) * N + gz) * CIN + (d >> 16) * 8;
+                __builtin_amdgcn_global_load_lds((const void*)gp, (__attribute__((address_space(3))) void*)(buf + f0 * 8), 16, 0, 0);
+            }
+        }
+    };
+
+    // this wavefront's weights: B fragments (rows (hgrp NHW + h) 16 + m, columns 32 ks + 8 g ..)
+    bf16x8 bw[KSTEPS][NHW];
+#pragma unroll
+    for (int ks = 0; ks < KSTEPS; ++ks) {
+        const int k0 = 32 * ks + 8 * g;
+#pragma unroll
+        for (int h = 0; h < NHW; ++h)
+            bw[ks][h] = (k0 < KT) ? ld16(a.w + ((hgrp * NHW + h) * 16 + m) * KT + k0) : bf16x8{};
+    }
+    float bias[NHW];
+#pragma unroll
+    for (int h = 0; h < NHW; ++h) bias[h] = a.bias[(hgrp * NHW + h) * 16 + m];
+    // halo z part of the A fragment offsets: z index m + dz, chunk cc (the rest is per z-row and tap)
+    int zoff[3][CH];
+#pragma unroll
+    for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+        for (int cc = 0; cc < CH; ++cc) zoff[dz][cc] = (m + dz) * CIN + ((cc ^ (((m + dz) >> SH) & (CH - 1))) * 8);
+
+    auto compute = [&](uint16_t* cur, int item) {
+        floatx4 acc[SUB][NHW];
+#pragma unroll
+        for (int s2 = 0; s2 < SUB; ++s2)
+#pragma unroll
+            for (int h = 0; h < NHW; ++h) acc[s2][h] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < KSTEPS; ++ks) {
+            bf16x8 af[SUB];
+#pragma unroll
+            for (int s2 = 0; s2 < SUB; ++s2) {
+                const int st = wrow * SUB + s2, ix = st / PY, iy = st % PY;
+                if constexpr (CIN == 32) {
+                    // K step ks = tap ks, channels 8 g ..: compile-time tap
+                    const int tap = ks, dx = tap / 9, dy = (tap / 3) % 3, dz = tap % 3;
+                    af[s2] = ld16(cur + ((ix + dx) * PHY + (iy + dy)) * PHZ * CIN + zoff[dz][g]);
+                } else {
+                    // K step ks = taps 2 ks (g < 2) and 2 ks + 1 (g >= 2), channels 8 (g % 2) ..
+                    const int t0 = 2 * ks, t1 = 2 * ks + 1;
+                    if (t1 < 27) {
+                        const int o0 = ((ix + t0 / 9) * PHY + (iy + (t0 / 3) % 3)) * PHZ * CIN + zoff[t0 % 3][g & 1];
+                        const int o1 = ((ix + t1 / 9) * PHY + (iy + (t1 / 3) % 3)) * PHZ * CIN + zoff[t1 % 3][g & 1];
+                        af[s2] = ld16(cur + (g < 2 ? o0 : o1));
+                    } else {
+                        const int o0 = ((ix + t0 / 9) * PHY + (iy + (t0 / 3) % 3)) * PHZ * CIN + zoff[t0 % 3][g & 1];
+                        af[s2] = (g < 2) ? ld16(cur + o0) : bf16x8{};  // K padding (27 taps = 13.5 steps)
+                    }
+                }
+            }
+#pragma unroll
+            for (int s2 = 0; s2 < SUB; ++s2)
+#pragma unroll
+                for (int h = 0; h < NHW; ++h)
+                    acc[s2][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[s2], bw[ks][h], acc[s2][h], 0, 0, 0);
+        }
+        // every wavefront is done reading cur: it becomes the output tile
+        asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+        int b, x0, y0, z0;
+        origin(item, b, x0, y0, z0);
+#pragma unroll
+        for (int s2 = 0; s2 < SUB; ++s2) {
+            const int st = wrow * SUB + s2, ix = st / PY, iy = st % PY;
+            const long long row = (long long)b * N3 + ((long long)(x0 + ix) * N + (y0 + iy)) * N + z0;
+#pragma unroll
+            for (int h = 0; h < NHW; ++h) {
+                const int co = (hgrp * NHW + h) * 16 + m;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const long long idx = (row + 4 * g + i) * COUT + co;
+                    const int tp = st * PZ + 4 * g + i;
+                    const float v = acc[s2][h][i] + bias[h];
+                    if constexpr (EPI == EPI_GELU) {
+                        cur[tp * OS + co] = f2bf(gelu(v));
+                    } else if constexpr (EPI == EPI_GELU_PRE) {
+                        a.f_out[idx] = v;
+                        cur[tp * OS + co] = f2bf(gelu(v));
+                    } else {
+                        a.f_out[idx] = v;
+                    }
+                }
+            }
+        }
+        if constexpr (EPI != EPI_LINEAR) {
+            asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+            constexpr int OC = COUT / 8;
+            for (int i = threadIdx.x; i < PPTS * OC; i += P_THREADS) {
+                const int tp = i / OC, c = i % OC;
+                const int iz = tp % PZ, iy = (tp / PZ) % PY, ix = tp / (PZ * PY);
+                const long long gp = (long long)b * N3 + ((long long)(x0 + ix) * N + (y0 + iy)) * N + (z0 + iz);
+                *(u32x4*)(a.act_out + gp * COUT + c * 8) = *(const u32x4*)(cur + tp * OS + c * 8);
+            }
+        }
+    };
+
+    int item = blockIdx.x;
+    if (item < nitems) issue(item, halo0);
+    while (item < nitems) {
+        // item in halo0; the next one (if any) goes to halo1
+        asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+        int nx = item + (int)gridDim.x;
+        if (nx < nitems) issue(nx, halo1);
+        compute(halo0, item);
+        item = nx;
+        if (item >= nitems) break;
+        asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+        nx = item + (int)gridDim.x;
+        if (nx < nitems) issue(nx, halo0);
+        compute(halo1, item);
+        item = nx;
+    }
+}
+
 // Weight gradients: partial[blk][co][tap * CIN + ci] = sum over this block's points p of
 // dp[p][co] * act[p + tap][ci], bpartial[blk][co] = sum dp[p][co].  GEMM view: M = COUT,
 // N = 27 CIN, K = points (32 per MFMA).  Persistent over (tile, sample); wavefront w owns the
@@ -423,7 +616,14 @@ int ntiles(const nsg_cnn* c) { return (c->N / TXT) * (c->N / TYT) * (c->N / TZT)
 
 template <int CIN, int COUT, int EPI>
 int launch_conv(nsg_cnn* c, hipStream_t s, int B, const ConvArgs& a) {
-    hipLaunchKernelGGL((conv3_kernel<CIN, COUT, EPI>), dim3(ntiles(c), 1, B), dim3(CONV_THREADS), 0, s, a);
+    if constexpr (EPI != EPI_BWD_GELU) {
+        const int nt = (c->N / PX) * (c->N / PY) * (c->N / PZ);
+        const int items = nt * B;
+        const int blocks = std::min(items, c->ncu);  // two halo buffers: one block per CU
+        hipLaunchKernelGGL((conv3p_kernel<CIN, COUT, EPI>), dim3(blocks), dim3(P_THREADS), 0, s, a, nt, items);
+    } else {
+        hipLaunchKernelGGL((conv3_kernel<CIN, COUT, EPI>), dim3(ntiles(c), 1, B), dim3(CONV_THREADS), 0, s, a);
+    }
     LCHK(c);
     return NSG_OK;
 }
@@ -629,6 +829,7 @@ int nsg_cnn_create(int N, int max_batch, uint64_t seed, int device, nsg_cnn** ou
         hipDeviceProp_t prop;
         CHK(c, hipGetDeviceProperties(&prop, device));
         c->wgrad_blocks = std::max(64, prop.multiProcessorCount);
+        c->ncu = std::max(1, prop.multiProcessorCount);
     }
     const long long np = (long long)max_batch * c->N3;
     auto alloc = [&](void** p, size_t bytes) -> int {

@@ -56,6 +56,7 @@ struct nsg_cnn {
     size_t partial_n = 0;
     double* lpart = nullptr;                 // loss partial sums
     int wgrad_blocks = 256;
+    int ncu = 256;                           // compute units of the device
     // data set (normalised): inputs bf16 [N^3][16], targets FP32 [N^3][6]
     std::vector<uint16_t*> xs;
     std::vector<float*> ys;
