@@ -128,6 +128,11 @@ inline void __syncthreads() {
     std::abort();
 }
 template <typename V> inline V __shfl(V v, int src) { return emu::shfl_from(v, src); }
+// global->LDS copies (pass 5's pipelined kernel) are not emulated: the z passes do not use them
+inline void __builtin_amdgcn_global_load_lds(const void*, void*, unsigned, int, int) {
+    std::fprintf(stderr, "emulation: global->LDS copies are not supported\n");
+    std::abort();
+}
 // the value of the first lane (every lane of an emulated wavefront is active)
 inline int __builtin_amdgcn_readfirstlane(int v) { return emu::shfl_from(v, 0); }
 template <typename V> inline V __shfl_xor(V v, int mask) { return emu::shfl_from(v, (int)(emu::t_tid.x & 63) ^ mask); }
