@@ -26,6 +26,24 @@ def test_shapiro_exact_decay_256(nsg):
         assert rel(s.get_state(), u0 * ((2 - D) / (2 + D)) ** 3) < 1e-12
 
 
+@pytest.mark.parametrize("scheme,precision", [("ifrk4", "fp64"), ("ifrk4", "fp32bulk"), ("rk4cn", "fp32bulk")])
+def test_shapiro_exact_decay_512(nsg, scheme, precision):
+    """the 512^3 kernels (pass 5 with its reads pipelined through LDS, the FP32 z pass pairing the
+    P_z of two lines) on the corrected Shapiro field, which is Beltrami: the nonlinear term is zero,
+    so three steps multiply the state by exactly exp(-nu lambda^2 dt)^3 (IF-RK4) or
+    ((2 - D)/(2 + D))^3 (RK4-CN), lambda^2 = 12, D = nu lambda^2 dt.  FP64 to 1e-12; FP32-bulk to
+    1e-6 (its transforms' round-off leaves a nonlinear term of that relative size)"""
+    N, nu, dt = 512, 1e-3, 1e-3
+    with nsg.Solver(N, nu=nu, dt=dt, scheme=scheme, precision=precision) as s:
+        s.init("shapiro")
+        u0 = s.get_state()
+        s.step(3)
+        got = s.get_state()
+    D = nu * 12 * dt
+    g = np.exp(-D) if scheme == "ifrk4" else (2 - D) / (2 + D)
+    assert rel(got, u0 * g ** 3) < (1e-12 if precision == "fp64" else 1e-6)
+
+
 def _half_weights(N):
     KX, KY, KZ = wavenumbers(N)
     return np.where((KZ == 0) | (KZ == N // 2), 1.0, 2.0), (KX, KY, KZ)
