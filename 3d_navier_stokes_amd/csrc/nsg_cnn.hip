@@ -43,13 +43,38 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
     return (uint16_t)(u >> 16);
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+// the same rounding (to nearest, ties to even) by gfx950's conversion instruction
+__device__ __forceinline__ uint16_t f2bf_hw(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 
 __device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+// GELU of the forward convolution's epilogue, x Phi(x) = x erfc(-x / sqrt 2) / 2, with erfc from
+// Abramowitz & Stegun 7.1.26 (erfc(z) = t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-z^2),
+// t = 1 / (1 + p z), z >= 0, absolute error <= 1.5e-7; erfc(-z) = 2 - erfc(z)), the hardware
+// reciprocal and exponential: one branch-free path, about a third of erff's instructions.  Its
+// difference from the erff form is far below half a bfloat16 unit of the rounded activation.
+__device__ __forceinline__ float gelu_as(float x) {
+    const float z = x * -0.70710678118654752f, a = fabsf(z);
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, a, 1.0f));
+    const float p = t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, 1.061405429f, -1.453152027f), 1.421413741f), -0.284496736f),
+                             0.254829592f);
+    const float e = p * __builtin_amdgcn_exp2f(a * a * -1.44269504088896341f);
+    return 0.5f * x * ((z >= 0.f) ? e : 2.0f - e);
+}
 __device__ __forceinline__ float gelu_grad(float x) {
     return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x);
 }
 
 __device__ __forceinline__ bf16x8 ld16(const uint16_t* p) { return __builtin_bit_cast(bf16x8, *(const u32x4*)p); }
+
+// 16 bytes per lane from global memory to LDS at lds + 16 lane (lds wave-uniform), written in
+// assembly so that the compiler does not track it: its own wait insertion treats any such copy as
+// possibly aliasing every LDS read and would make the MFMA loop wait for the copies of the next
+// item; the kernel waits for them itself.  No compiler-generated code in the kernels that use it
+// keeps a value in M0.
+__device__ __forceinline__ void dma16(const void* g, const uint16_t* lds) {
+    const uint32_t off = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) const void*)lds;
+    asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" : : "v"(g), "s"(off) : "memory");
+}
 
 __device__ __forceinline__ int wrap(int v, int N) { return v < 0 ? v + N : (v >= N ? v - N : v); }
 
@@ -273,7 +298,7 @@ __global__ void __launch_bounds__(P_THREADS, 2) conv3p_kernel(ConvArgs a, int nt
                 const int gx = wrap(x0 + (d & 15) - 1, N), gy = wrap(y0 + ((d >> 4) & 15) - 1, N);
                 const int gz = wrap(z0 + ((d >> 8) & 31) - 1, N);
                 const uint16_t* gp = in + (((long long)gx * N + gy) * N + gz) * CIN + (d >> 16) * 8;
-                __builtin_amdgcn_global_load_lds((const void*)gp, (__attribute__((address_space(3))) void*)(buf + f0 * 8), 16, 0, 0);
+                dma16((const void*)gp, buf + f0 * 8);
             }
         }
     };
@@ -349,10 +374,10 @@ __global__ void __launch_bounds__(P_THREADS, 2) conv3p_kernel(ConvArgs a, int nt
                     const int tp = st * PZ + 4 * g + i;
                     const float v = acc[s2][h][i] + bias[h];
                     if constexpr (EPI == EPI_GELU) {
-                        cur[tp * OS + co] = f2bf(gelu(v));
+                        cur[tp * OS + co] = f2bf_hw(gelu_as(v));
                     } else if constexpr (EPI == EPI_GELU_PRE) {
                         a.f_out[idx] = v;
-                        cur[tp * OS + co] = f2bf(gelu(v));
+                        cur[tp * OS + co] = f2bf_hw(gelu_as(v));
                     } else {
                         a.f_out[idx] = v;
                     }
@@ -371,17 +396,28 @@ __global__ void __launch_bounds__(P_THREADS, 2) conv3p_kernel(ConvArgs a, int nt
         }
     };
 
+    // Before an item: this wavefront's copies of its halo have landed (and its LDS accesses are
+    // done), then every wavefront's.  The previous item's global stores, issued after those copies,
+    // may stay in flight: the vector-memory counter completes in issue order, so waiting until at
+    // most NST operations are outstanding (the stores one compute issues) covers the copies.  The
+    // first item also waits for the weight and bias reads.
+    constexpr int NST = ((EPI != EPI_LINEAR) ? PPTS * (COUT / 8) / P_THREADS : 0) + ((EPI != EPI_GELU) ? SUB * NHW * 4 : 0);
+    static_assert(NST < 64, "vmcnt field");
+    constexpr int WAIT_NST = (NST & 15) | (7 << 4) | ((NST >> 4) << 14);  // vmcnt(NST) lgkmcnt(0)
     int item = blockIdx.x;
     if (item < nitems) issue(item, halo0);
+    asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory");
     while (item < nitems) {
         // item in halo0; the next one (if any) goes to halo1
-        asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+        __builtin_amdgcn_s_waitcnt(WAIT_NST);
+        asm volatile("s_barrier" ::: "memory");
         int nx = item + (int)gridDim.x;
         if (nx < nitems) issue(nx, halo1);
         compute(halo0, item);
         item = nx;
         if (item >= nitems) break;
-        asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+        __builtin_amdgcn_s_waitcnt(WAIT_NST);
+        asm volatile("s_barrier" ::: "memory");
         nx = item + (int)gridDim.x;
         if (nx < nitems) issue(nx, halo0);
         compute(halo1, item);
