@@ -42,23 +42,25 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
     u += 0x7fffu + ((u >> 16) & 1u);
     return (uint16_t)(u >> 16);
 }
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 // the same rounding (to nearest, ties to even) by gfx950's conversion instruction
 __device__ __forceinline__ uint16_t f2bf_hw(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 
-__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
-// GELU of the forward convolution's epilogue, x Phi(x) = x erfc(-x / sqrt 2) / 2, with erfc from
-// Abramowitz & Stegun 7.1.26 (erfc(z) = t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-z^2),
-// t = 1 / (1 + p z), z >= 0, absolute error <= 1.5e-7; erfc(-z) = 2 - erfc(z)), the hardware
-// reciprocal and exponential: one branch-free path, about a third of erff's instructions.  Its
-// difference from the erff form is far below half a bfloat16 unit of the rounded activation.
+// GELU of the forward convolution's epilogue, x Phi(x) = max(x, 0) - |x| erfc(|x| / sqrt 2) / 2,
+// with erfc from Abramowitz & Stegun 7.1.26 (erfc(z) = t (a1 + t (a2 + t (a3 + t (a4 + t a5))))
+// exp(-z^2), t = 1 / (1 + p z), absolute error <= 1.5e-7) written in w = x sqrt(log2(e) / 2), so
+// that exp(-z^2) = exp2(-w^2) and p z = p' |w| with p' = p / sqrt(log2 e); the halves are folded
+// into the coefficients.  One branch-free path of 14 instructions (hardware reciprocal and
+// exponential); its difference from the erff form is far below half a bfloat16 unit of the
+// rounded activation.
 __device__ __forceinline__ float gelu_as(float x) {
-    const float z = x * -0.70710678118654752f, a = fabsf(z);
-    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, a, 1.0f));
-    const float p = t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, 1.061405429f, -1.453152027f), 1.421413741f), -0.284496736f),
-                             0.254829592f);
-    const float e = p * __builtin_amdgcn_exp2f(a * a * -1.44269504088896341f);
-    return 0.5f * x * ((z >= 0.f) ? e : 2.0f - e);
+    const float w = x * 0.84932180028801907f;
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.27273748087922250f, fabsf(w), 1.0f));
+    const float q = t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, 0.5307027145f, -0.7265760135f), 0.7107068705f), -0.142248368f),
+                             0.127414796f);
+    const float e = __builtin_amdgcn_exp2f(-(w * w));
+    return fmaf(-fabsf(x), q * e, fmaxf(x, 0.0f));
 }
 __device__ __forceinline__ float gelu_grad(float x) {
     return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x);
@@ -74,6 +76,13 @@ __device__ __forceinline__ bf16x8 ld16(const uint16_t* p) { return __builtin_bit
 __device__ __forceinline__ void dma16(const void* g, const uint16_t* lds) {
     const uint32_t off = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) const void*)lds;
     asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" : : "v"(g), "s"(off) : "memory");
+}
+
+// the same copy from base + off bytes (base wave-uniform, off < 2^32): the address is formed by
+// the instruction (scalar base, 32-bit vector offset)
+__device__ __forceinline__ void dma16o(const void* base, unsigned off, const uint16_t* lds) {
+    const uint32_t l = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) const void*)lds;
+    asm volatile("s_mov_b32 m0, %2\n\tglobal_load_lds_dwordx4 %0, %1" : : "v"(off), "s"(base), "s"(l) : "memory");
 }
 
 __device__ __forceinline__ int wrap(int v, int N) { return v < 0 ? v + N : (v >= N ? v - N : v); }
@@ -221,41 +230,41 @@ __global__ void __launch_bounds__(CONV_THREADS) conv3_kernel(ConvArgs a) {
     }
 }
 
-// Forward convolution, persistent and pipelined (forward modes).  A 512-thread block walks the
-// (sample, tile) items of 4 x 4 x 16 output points; the 6 x 6 x 18 periodic halo of the next item
-// goes from HBM to LDS through the global->LDS DMA (64 lanes x 16 bytes per instruction) while the
-// MFMAs of the current item run, into the other of two halo buffers.  The layer's weights stay in
-// registers for the whole kernel: with 32 output channels the wavefronts split them (waves 0-3
-// hold channels 0-15, waves 4-7 channels 16-31, each over 4 z-rows), with 16 every wavefront holds
-// them all (over 2 z-rows), so LDS carries only the A operand and two wavefronts share a SIMD.
-// The DMA writes each halo point's CIN channels contiguously (no padding), so the 16-byte chunks
-// of a point are swizzled instead: chunk c of the point with halo z index hz sits in slot
-// c ^ ((hz >> SH) & (CH - 1)), which puts the 8 points that one LDS cycle serves in distinct
-// banks.  GEMM view per item as conv3_kernel's: M = 256 points (16 z-rows), N = COUT, K = 27 taps
-// x CIN in steps of 32.  The loop over items is unrolled by two so that the two halo buffers are
-// distinct LDS objects: the reads of one do not wait for the copies into the other.
+// Forward convolution (forward modes), persistent.  Blocks of 256 threads walk the (sample, tile)
+// items of 4 x 4 x 16 output points, two blocks per CU.  The 6 x 6 x 18 periodic halo of an item
+// goes from HBM to LDS through the global->LDS DMA (64 lanes x 16 bytes per instruction); a block
+// has one halo buffer and a separate output tile, and copies the next item's halo as soon as every
+// wavefront has read the current one, while its own epilogue (GELU, output tile, stores) runs; the
+// two blocks of a CU drift apart, so one's epilogue and copies overlap the other's MFMAs.  The
+// layer's weights stay in registers for the whole kernel: a wavefront holds 16 output channels'
+// (with COUT = 32 the four wavefronts split them into two halves over 8 z-rows each, with 16 each
+// holds them all over 4 z-rows), so LDS carries only the A operand.  The DMA writes each halo
+// point's CIN channels contiguously (no padding), so the 16-byte chunks of a point are swizzled
+// instead: chunk c of the point with halo z index hz sits in slot c ^ ((hz >> SH) & (CH - 1)),
+// which puts the 8 points that one LDS cycle serves in distinct banks.  GEMM view per item as
+// conv3_kernel's: M = 256 points (16 z-rows), N = COUT, K = 27 taps x CIN in steps of 32.
 constexpr int PX = 4, PY = 4, PZ = 16;
 constexpr int PHY = PY + 2, PHZ = PZ + 2, PHPTS = (PX + 2) * PHY * PHZ, PPTS = PX * PY * PZ;
-constexpr int P_THREADS = 512;
+constexpr int Q_THREADS = 256;
 
 template <int CIN, int COUT, int EPI>
-__global__ void __launch_bounds__(P_THREADS, 2) conv3p_kernel(ConvArgs a, int ntile, int nitems) {
+__global__ void __launch_bounds__(Q_THREADS, 2) conv3p_kernel(ConvArgs a, int ntile, int nitems) {
     static_assert(EPI == EPI_GELU || EPI == EPI_GELU_PRE || EPI == EPI_LINEAR, "forward modes only");
     constexpr int CH = CIN / 8;                  // 16-byte chunks per point
     constexpr int SH = (CH == 4) ? 1 : 2;        // swizzle shift (CH = 2 or 4)
     static_assert(CH == 2 || CH == 4, "CIN = 16 or 32");
     constexpr int KT = 27 * CIN, KSTEPS = (KT + 31) / 32, NH = COUT / 16;
-    constexpr int NWAVE = P_THREADS / 64;
+    constexpr int NWAVE = Q_THREADS / 64;
     constexpr int HS = (NH >= 2) ? 2 : 1;        // channel groups of wavefronts
     constexpr int NHW = NH / HS;                 // 16-channel blocks per wavefront
     constexpr int SUB = PPTS / 16 / (NWAVE / HS);  // z-rows per wavefront
     constexpr int OS = COUT + 8;                 // output tile point stride (halfwords)
-    constexpr int HB = PHPTS * CIN;              // halfwords per halo buffer
+    constexpr int HB = PHPTS * CIN;              // halfwords of the halo buffer
     constexpr int NCHUNK = PHPTS * CH;
-    constexpr int ROUNDS = (NCHUNK + P_THREADS - 1) / P_THREADS;
-    static_assert(PPTS * OS <= HB, "output tile must fit in a halo buffer");
-    __shared__ __attribute__((aligned(16))) uint16_t halo0[HB];
-    __shared__ __attribute__((aligned(16))) uint16_t halo1[HB];
+    constexpr int ROUNDS = (NCHUNK + Q_THREADS - 1) / Q_THREADS;
+    constexpr int OT = (EPI != EPI_LINEAR) ? PPTS * OS : 8;  // bf16 output tile (GELU modes)
+    __shared__ __attribute__((aligned(16))) uint16_t halo[HB];
+    __shared__ __attribute__((aligned(16))) uint16_t outt[OT];
     const int N = a.N;
     const long long N3 = (long long)N * N * N;
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -263,18 +272,28 @@ __global__ void __launch_bounds__(P_THREADS, 2) conv3p_kernel(ConvArgs a, int nt
     const int wrow = wv % (NWAVE / HS), hgrp = wv / (NWAVE / HS);  // z-rows wrow SUB .., channels hgrp NHW 16 ..
 
     // this thread's share of a halo copy, the same for every item: chunk f = (r NWAVE + wv) 64 + lane
-    // of point (hx, hy, hz) and source chunk src, packed as hx | hy << 4 | hz << 8 | src << 16
-    // (-1: past the halo)
-    int dmac[ROUNDS];
+    // is chunk src of halo point (hx, hy, hz), held as its byte offset from the point (x0 - 1,
+    // y0 - 1, z0 - 1) without wrapping, 2 (((hx N + hy) N + hz) CIN + 8 src) (bits 0-25, N <= 256),
+    // with flags for the halo faces that may wrap: hx = 0 (bit 26), hx = PX + 1 (27), hy = 0 (28),
+    // hy = PY + 1 (29), hz = 0 (30), hz = PZ + 1 (31); ~0: past the halo.  Kept in LDS (each
+    // thread reads back only its own entries): in registers they would spill, and a spill reload
+    // waits for every copy in flight.
+    const bool off32 = N <= 256;  // sample offsets fit 32 bits: 2 N^3 CIN bytes < 2^31
+    constexpr int RFULL = NCHUNK / (NWAVE * 64);  // rounds with every lane inside the halo
+    __shared__ unsigned hoff_t[ROUNDS * Q_THREADS];
+    unsigned* hoff = hoff_t + threadIdx.x;  // entry r at hoff[r Q_THREADS]
 #pragma unroll
     for (int r = 0; r < ROUNDS; ++r) {
         const int f = (r * NWAVE + wv) * 64 + lane;
         if (f < NCHUNK) {
             const int q = f / CH, slot = f % CH;
             const int hz = q % PHZ, hy = (q / PHZ) % PHY, hx = q / (PHZ * PHY);
-            dmac[r] = hx | (hy << 4) | (hz << 8) | ((slot ^ ((hz >> SH) & (CH - 1))) << 16);
+            const int src = slot ^ ((hz >> SH) & (CH - 1));
+            hoff[r * Q_THREADS] = (unsigned)((2 * (((hx * N + hy) * N + hz) * CIN + src * 8)) & 0x3ffffff) |
+                      ((unsigned)(hx == 0) << 26) | ((unsigned)(hx == PX + 1) << 27) | ((unsigned)(hy == 0) << 28) |
+                      ((unsigned)(hy == PY + 1) << 29) | ((unsigned)(hz == 0) << 30) | ((unsigned)(hz == PZ + 1) << 31);
         } else {
-            dmac[r] = -1;
+            hoff[r * Q_THREADS] = ~0u;
         }
     }
     auto origin = [&](int item, int& b, int& x0, int& y0, int& z0) {
@@ -286,24 +305,67 @@ __global__ void __launch_bounds__(P_THREADS, 2) conv3p_kernel(ConvArgs a, int nt
         y0 = (t % nty) * PY;
         x0 = (t / nty) * PX;
     };
-    auto issue = [&](int item, uint16_t* buf) {
+    auto issue = [&](int item) {
         int b, x0, y0, z0;
         origin(item, b, x0, y0, z0);
         const uint16_t* in = a.in + (long long)b * N3 * CIN;
+        if (off32) {
+            // byte offset = the item's corner (x0 - 1, y0 - 1, z0 - 1), possibly negative, + the
+            // lane's unwrapped offset + N along each axis whose face lies outside the box for this
+            // item (0 or -1 masks from the face flags); three loops, by the axes that wrap
+            const int ib = 2 * ((((x0 - 1) * N + (y0 - 1)) * N + (z0 - 1)) * CIN);
+            const bool wxl = x0 == 0, wxh = x0 + PX == N, wyl = y0 == 0, wyh = y0 + PY == N;
+            const bool wzl = z0 == 0, wzh = z0 + PZ == N;
+            const int sx = 2 * N * N * CIN, sy = 2 * N * CIN, sz = 2 * CIN;
+            auto one = [&](int r, unsigned d, unsigned off) {
+                if (r < RFULL || d != ~0u) dma16o(in, off, halo + (r * NWAVE + wv) * 64 * 8);
+            };
+            if (!(wxl || wxh || wyl || wyh || wzl || wzh)) {
 #pragma unroll
-        for (int r = 0; r < ROUNDS; ++r) {
-            const int f0 = (r * NWAVE + wv) * 64;  // uniform
-            const int d = dmac[r];
-            if (d >= 0) {
-                const int gx = wrap(x0 + (d & 15) - 1, N), gy = wrap(y0 + ((d >> 4) & 15) - 1, N);
-                const int gz = wrap(z0 + ((d >> 8) & 31) - 1, N);
-                const uint16_t* gp = in + (((long long)gx * N + gy) * N + gz) * CIN + (d >> 16) * 8;
-                dma16((const void*)gp, buf + f0 * 8);
+                for (int r = 0; r < ROUNDS; ++r) {
+                    const unsigned d = hoff[r * Q_THREADS];
+                    one(r, d, (d & 0x3ffffffu) + (unsigned)ib);
+                }
+            } else if (!(wxl || wxh || wyl || wyh) && wzl != wzh) {
+                const int bz = wzl ? 30 : 31, cz = (wzl ? N : -N) * sz;
+#pragma unroll
+                for (int r = 0; r < ROUNDS; ++r) {
+                    const unsigned d = hoff[r * Q_THREADS];
+                    one(r, d, (d & 0x3ffffffu) + (unsigned)ib + (unsigned)(__builtin_amdgcn_sbfe((int)d, bz, 1) & cz));
+                }
+            } else {
+                const int cxl = wxl ? N * sx : 0, cxh = wxh ? -N * sx : 0, cyl = wyl ? N * sy : 0;
+                const int cyh = wyh ? -N * sy : 0, czl = wzl ? N * sz : 0, czh = wzh ? -N * sz : 0;
+#pragma unroll
+                for (int r = 0; r < ROUNDS; ++r) {
+                    const unsigned d = hoff[r * Q_THREADS];
+                    const int c = (__builtin_amdgcn_sbfe((int)d, 26, 1) & cxl) + (__builtin_amdgcn_sbfe((int)d, 27, 1) & cxh) +
+                                  (__builtin_amdgcn_sbfe((int)d, 28, 1) & cyl) + (__builtin_amdgcn_sbfe((int)d, 29, 1) & cyh) +
+                                  (__builtin_amdgcn_sbfe((int)d, 30, 1) & czl) + (__builtin_amdgcn_sbfe((int)d, 31, 1) & czh);
+                    one(r, d, (d & 0x3ffffffu) + (unsigned)ib + (unsigned)c);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < ROUNDS; ++r) {
+                const int f0 = (r * NWAVE + wv) * 64;
+                int f = f0 + lane;
+                asm volatile("" : "+v"(f));
+                if (f < NCHUNK) {
+                    const int q = f / CH, slot = f % CH;
+                    const int hz = q % PHZ, hy = (q / PHZ) % PHY, hx = q / (PHZ * PHY);
+                    const int src = slot ^ ((hz >> SH) & (CH - 1));
+                    const int gx = wrap(x0 + hx - 1, N), gy = wrap(y0 + hy - 1, N), gz = wrap(z0 + hz - 1, N);
+                    const uint16_t* gp = in + (((long long)gx * N + gy) * N + gz) * CIN + src * 8;
+                    dma16((const void*)gp, halo + f0 * 8);
+                }
             }
         }
     };
 
-    // this wavefront's weights: B fragments (rows (hgrp NHW + h) 16 + m, columns 32 ks + 8 g ..)
+    // weights of K step ks: tap dx 9 + dy 3 + dz of the step's (dx, dz, dy) order for CIN = 32,
+    // taps 2 ks, 2 ks + 1 for CIN = 16
+    auto kmap = [](int ks) { return (CIN == 32) ? (ks / 9) * 9 + (ks % 3) * 3 + (ks / 3) % 3 : ks; };
     bf16x8 bw[KSTEPS][NHW];
 #pragma unroll
     for (int ks = 0; ks < KSTEPS; ++ks) {
@@ -315,14 +377,29 @@ __global__ void __launch_bounds__(P_THREADS, 2) conv3p_kernel(ConvArgs a, int nt
     float bias[NHW];
 #pragma unroll
     for (int h = 0; h < NHW; ++h) bias[h] = a.bias[(hgrp * NHW + h) * 16 + m];
-    // halo z part of the A fragment offsets: z index m + dz, chunk cc (the rest is per z-row and tap)
-    int zoff[3][CH];
+    // halo z part of this lane's A fragment offsets: z index m + dz, chunk g (CIN = 32) or g % 2
+    // (CIN = 16: lanes g >= 2 read the second tap of a K step)
+    int zoff[3];
 #pragma unroll
-    for (int dz = 0; dz < 3; ++dz)
-#pragma unroll
-        for (int cc = 0; cc < CH; ++cc) zoff[dz][cc] = (m + dz) * CIN + ((cc ^ (((m + dz) >> SH) & (CH - 1))) * 8);
+    for (int dz = 0; dz < 3; ++dz) {
+        const int cc = (CH == 4) ? g : (g & 1);
+        zoff[dz] = (m + dz) * CIN + ((cc ^ (((m + dz) >> SH) & (CH - 1))) * 8);
+    }
 
-    auto compute = [&](uint16_t* cur, int item) {
+    // Before an item: this wavefront's copies of its halo have landed (and its LDS accesses are
+    // done), then every wavefront's.  The previous item's global stores, issued after those copies,
+    // may stay in flight: the vector-memory counter completes in issue order, so waiting until at
+    // most NST operations are outstanding (the stores one item issues) covers the copies.  The
+    // first item also waits for the weight and bias reads.
+    constexpr int NST = ((EPI != EPI_LINEAR) ? PPTS * (COUT / 8) / Q_THREADS : 0) + ((EPI != EPI_GELU) ? SUB * NHW * 4 : 0);
+    static_assert(NST < 64, "vmcnt field");
+    constexpr int WAIT_NST = (NST & 15) | (7 << 4) | ((NST >> 4) << 14);
+    int item = blockIdx.x;
+    if (item < nitems) issue(item);
+    asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory");
+    while (item < nitems) {
+        __builtin_amdgcn_s_waitcnt(WAIT_NST);
+        asm volatile("s_barrier" ::: "memory");
         floatx4 acc[SUB][NHW];
 #pragma unroll
         for (int s2 = 0; s2 < SUB; ++s2)
@@ -335,19 +412,19 @@ __global__ void __launch_bounds__(P_THREADS, 2) conv3p_kernel(ConvArgs a, int nt
             for (int s2 = 0; s2 < SUB; ++s2) {
                 const int st = wrow * SUB + s2, ix = st / PY, iy = st % PY;
                 if constexpr (CIN == 32) {
-                    // K step ks = tap ks, channels 8 g ..: compile-time tap
-                    const int tap = ks, dx = tap / 9, dy = (tap / 3) % 3, dz = tap % 3;
-                    af[s2] = ld16(cur + ((ix + dx) * PHY + (iy + dy)) * PHZ * CIN + zoff[dz][g]);
+                    // K step ks = tap kmap(ks): the three dy taps of one (dx, dz) are consecutive,
+                    // so a fragment shared by neighbouring z-rows is live for three steps only
+                    const int dx = ks / 9, dz = (ks / 3) % 3, dy = ks % 3;
+                    af[s2] = ld16(halo + ((ix + dx) * PHY + (iy + dy)) * PHZ * CIN + zoff[dz]);
                 } else {
-                    // K step ks = taps 2 ks (g < 2) and 2 ks + 1 (g >= 2), channels 8 (g % 2) ..
                     const int t0 = 2 * ks, t1 = 2 * ks + 1;
                     if (t1 < 27) {
-                        const int o0 = ((ix + t0 / 9) * PHY + (iy + (t0 / 3) % 3)) * PHZ * CIN + zoff[t0 % 3][g & 1];
-                        const int o1 = ((ix + t1 / 9) * PHY + (iy + (t1 / 3) % 3)) * PHZ * CIN + zoff[t1 % 3][g & 1];
-                        af[s2] = ld16(cur + (g < 2 ? o0 : o1));
+                        const int o0 = ((ix + t0 / 9) * PHY + (iy + (t0 / 3) % 3)) * PHZ * CIN + zoff[t0 % 3];
+                        const int o1 = ((ix + t1 / 9) * PHY + (iy + (t1 / 3) % 3)) * PHZ * CIN + zoff[t1 % 3];
+                        af[s2] = ld16(halo + (g < 2 ? o0 : o1));
                     } else {
-                        const int o0 = ((ix + t0 / 9) * PHY + (iy + (t0 / 3) % 3)) * PHZ * CIN + zoff[t0 % 3][g & 1];
-                        af[s2] = (g < 2) ? ld16(cur + o0) : bf16x8{};  // K padding (27 taps = 13.5 steps)
+                        const int o0 = ((ix + t0 / 9) * PHY + (iy + (t0 / 3) % 3)) * PHZ * CIN + zoff[t0 % 3];
+                        af[s2] = (g < 2) ? ld16(halo + o0) : bf16x8{};
                     }
                 }
             }
@@ -355,10 +432,12 @@ __global__ void __launch_bounds__(P_THREADS, 2) conv3p_kernel(ConvArgs a, int nt
             for (int s2 = 0; s2 < SUB; ++s2)
 #pragma unroll
                 for (int h = 0; h < NHW; ++h)
-                    acc[s2][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[s2], bw[ks][h], acc[s2][h], 0, 0, 0);
+                    acc[s2][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[s2], bw[kmap(ks)][h], acc[s2][h], 0, 0, 0);
         }
-        // every wavefront is done reading cur: it becomes the output tile
+        // every wavefront is done reading the halo: the next item's copy may overwrite it
         asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+        const int nx = item + (int)gridDim.x;
+        if (nx < nitems) issue(nx);
         int b, x0, y0, z0;
         origin(item, b, x0, y0, z0);
 #pragma unroll
@@ -374,10 +453,10 @@ __global__ void __launch_bounds__(P_THREADS, 2) conv3p_kernel(ConvArgs a, int nt
                     const int tp = st * PZ + 4 * g + i;
                     const float v = acc[s2][h][i] + bias[h];
                     if constexpr (EPI == EPI_GELU) {
-                        cur[tp * OS + co] = f2bf_hw(gelu_as(v));
+                        outt[tp * OS + co] = f2bf_hw(gelu_as(v));
                     } else if constexpr (EPI == EPI_GELU_PRE) {
                         a.f_out[idx] = v;
-                        cur[tp * OS + co] = f2bf_hw(gelu_as(v));
+                        outt[tp * OS + co] = f2bf_hw(gelu_as(v));
                     } else {
                         a.f_out[idx] = v;
                     }
@@ -387,40 +466,15 @@ __global__ void __launch_bounds__(P_THREADS, 2) conv3p_kernel(ConvArgs a, int nt
         if constexpr (EPI != EPI_LINEAR) {
             asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
             constexpr int OC = COUT / 8;
-            for (int i = threadIdx.x; i < PPTS * OC; i += P_THREADS) {
+#pragma unroll
+            for (int j = 0; j < PPTS * OC / Q_THREADS; ++j) {
+                const int i = j * Q_THREADS + threadIdx.x;
                 const int tp = i / OC, c = i % OC;
                 const int iz = tp % PZ, iy = (tp / PZ) % PY, ix = tp / (PZ * PY);
                 const long long gp = (long long)b * N3 + ((long long)(x0 + ix) * N + (y0 + iy)) * N + (z0 + iz);
-                *(u32x4*)(a.act_out + gp * COUT + c * 8) = *(const u32x4*)(cur + tp * OS + c * 8);
+                *(u32x4*)(a.act_out + gp * COUT + c * 8) = *(const u32x4*)(outt + tp * OS + c * 8);
             }
         }
-    };
-
-    // Before an item: this wavefront's copies of its halo have landed (and its LDS accesses are
-    // done), then every wavefront's.  The previous item's global stores, issued after those copies,
-    // may stay in flight: the vector-memory counter completes in issue order, so waiting until at
-    // most NST operations are outstanding (the stores one compute issues) covers the copies.  The
-    // first item also waits for the weight and bias reads.
-    constexpr int NST = ((EPI != EPI_LINEAR) ? PPTS * (COUT / 8) / P_THREADS : 0) + ((EPI != EPI_GELU) ? SUB * NHW * 4 : 0);
-    static_assert(NST < 64, "vmcnt field");
-    constexpr int WAIT_NST = (NST & 15) | (7 << 4) | ((NST >> 4) << 14);  // vmcnt(NST) lgkmcnt(0)
-    int item = blockIdx.x;
-    if (item < nitems) issue(item, halo0);
-    asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory");
-    while (item < nitems) {
-        // item in halo0; the next one (if any) goes to halo1
-        __builtin_amdgcn_s_waitcnt(WAIT_NST);
-        asm volatile("s_barrier" ::: "memory");
-        int nx = item + (int)gridDim.x;
-        if (nx < nitems) issue(nx, halo1);
-        compute(halo0, item);
-        item = nx;
-        if (item >= nitems) break;
-        __builtin_amdgcn_s_waitcnt(WAIT_NST);
-        asm volatile("s_barrier" ::: "memory");
-        nx = item + (int)gridDim.x;
-        if (nx < nitems) issue(nx, halo0);
-        compute(halo1, item);
         item = nx;
     }
 }
@@ -655,8 +709,8 @@ int launch_conv(nsg_cnn* c, hipStream_t s, int B, const ConvArgs& a) {
     if constexpr (EPI != EPI_BWD_GELU) {
         const int nt = (c->N / PX) * (c->N / PY) * (c->N / PZ);
         const int items = nt * B;
-        const int blocks = std::min(items, c->ncu);  // two halo buffers: one block per CU
-        hipLaunchKernelGGL((conv3p_kernel<CIN, COUT, EPI>), dim3(blocks), dim3(P_THREADS), 0, s, a, nt, items);
+        const int blocks = std::min(items, 2 * c->ncu);  // two blocks per CU
+        hipLaunchKernelGGL((conv3p_kernel<CIN, COUT, EPI>), dim3(blocks), dim3(Q_THREADS), 0, s, a, nt, items);
     } else {
         hipLaunchKernelGGL((conv3_kernel<CIN, COUT, EPI>), dim3(ntiles(c), 1, B), dim3(CONV_THREADS), 0, s, a);
     }
