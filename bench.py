@@ -265,20 +265,28 @@ def run_closure(nsg, dns, r_dns, Nl=64, steps=200, warmup=20):
     dt_les = (les.time()[0] - tb) / steps
     ms_fwd = net.time_forward(1, 20)
     les.close()
+    net.close()
+    # the same forward over a training batch of 8 samples (more items per CU than one sample's)
+    net8 = nsg.Network(Nl, max_batch=8, seed=1)
+    ms_fwd8 = net8.time_forward(8, 10)
+    net8.close()
     flop = 2.0 * Nl ** 3 * 27 * sum(a * b for a, b in zip(CNN_CH[:-1], CNN_CH[1:]))
     tflops = flop / (ms_fwd * 1e-3) / 1e12
+    tflops8 = 8 * flop / (ms_fwd8 * 1e-3) / 1e12
     dns_s_per_tu = r_dns["ms_per_step"] * 1e-3 / r_dns["mean_dt"]
     les_s_per_tu = el / steps / dt_les
     out = {"les_grid": Nl, "les_ms_per_step": 1e3 * el / steps, "les_mean_dt": dt_les,
            "dns_ms_per_step": r_dns["ms_per_step"], "dns_mean_dt": r_dns["mean_dt"],
            "gpu_time_ratio_dns_over_les": dns_s_per_tu / les_s_per_tu,
            "cnn_forward_ms": ms_fwd, "cnn_forward_GFLOP": flop / 1e9,
-           "roofline": {"bound": "mfma", "kernel": "conv3_kernel (4 layers, one forward)", "achieved": round(tflops, 2),
-                        "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(tflops / BF16_PEAK_TFLOPS, 5),
-                        "traffic": None}}
+           "roofline": {"bound": "mfma", "kernel": "conv3p_kernel (4 layers, one forward, 1 sample)",
+                        "achieved": round(tflops, 2), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(tflops / BF16_PEAK_TFLOPS, 5), "traffic": None},
+           "cnn_forward_batch8": {"ms": ms_fwd8, "achieved_TFLOPs": round(tflops8, 2),
+                                  "frac": round(tflops8 / BF16_PEAK_TFLOPS, 5)}}
     log(f"[bench] closure LES {Nl}^3: {out['les_ms_per_step']:.3f} ms/step, GPU-time ratio DNS/LES "
-        f"{out['gpu_time_ratio_dns_over_les']:.0f}, CNN forward {ms_fwd:.3f} ms ({tflops:.1f} TFLOP/s)")
-    net.close()
+        f"{out['gpu_time_ratio_dns_over_les']:.0f}, CNN forward {ms_fwd:.3f} ms ({tflops:.1f} TFLOP/s), "
+        f"batch 8 {ms_fwd8:.3f} ms ({tflops8:.1f} TFLOP/s)")
     return out
 
 
