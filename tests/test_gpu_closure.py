@@ -47,7 +47,9 @@ def test_param_count(nsg):
         assert np.array_equal(net.get_params(), q)
 
 
-@pytest.mark.parametrize("N,B", [(16, 1), (16, 2), (32, 1), (64, 1)])
+# N = 16: a tile's halo wraps on both z faces; 32: every tile wraps somewhere; 48 and 64: tiles
+# inside the box (no wrap), z wraps only, and x / y wraps all occur
+@pytest.mark.parametrize("N,B", [(16, 1), (16, 2), (32, 1), (48, 1), (64, 1)])
 def test_forward_vs_oracle(nsg, N, B):
     x, _ = rand_inputs(10 + N + B, B, N)
     with nsg.Network(N, max_batch=B, seed=5) as net:
