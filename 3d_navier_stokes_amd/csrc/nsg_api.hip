@@ -377,8 +377,12 @@ static int launch_pass5(nsg_ctx* c, double2* uh, int b0, int nb, int stage, void
             const int ntiles = c->NZP / W;
             const long long nitems = (long long)ntiles * c->NK * nb;
             const int blocks = (int)std::min<long long>(nitems, c->ncu);  // 2 N W elements of LDS: one block per CU
-            hipLaunchKernelGGL((pass5p_kernel<T, N, EPT, W>), dim3((unsigned)blocks), dim3(TPF * W), 0, c->s, a, ntiles,
-                               (int)nitems);
+            if (c->p.scheme == RK4CN)
+                hipLaunchKernelGGL((pass5p_kernel<T, N, EPT, W, RK4CN>), dim3((unsigned)blocks), dim3(TPF * W), 0, c->s, a,
+                                   ntiles, (int)nitems);
+            else
+                hipLaunchKernelGGL((pass5p_kernel<T, N, EPT, W, IFRK4>), dim3((unsigned)blocks), dim3(TPF * W), 0, c->s, a,
+                                   ntiles, (int)nitems);
             LAUNCHCHK(c);
             prof_end(c, 4);
             return NSG_OK;

@@ -1183,14 +1183,35 @@ constexpr double DPE1 = DPB1 - 5179.0 / 57600.0, DPE3 = DPB3 - 7571.0 / 16695.0,
 // error tolerances (Solver/src/data_types.h:123-124)
 constexpr double DP_ATOL = 1e-7, DP_RTOL = 1e-7;
 
+// Reads of the final RK4 update of slot m (state and K1..K3 of the three components), when the slot
+// is stored.
+template <typename T, int N, int EPT, bool O32>
+__device__ __forceinline__ void p5_fetch(int m, int t, int iy, int kz, int kmax, int NK, int NZP, long long plane,
+                                         const double2* uh, const cpx_t<T>* K1, const cpx_t<T>* K2, const cpx_t<T>* K3,
+                                         double2 (&u0)[3], cpx_t<T> (&q1)[3], cpx_t<T> (&q2)[3], cpx_t<T> (&q3)[3]) {
+    const int kx = k_of_pos(t + (N / EPT) * m, N);
+    if (kx < -kmax || kx > kmax || kz >= NZP) return;
+    const long long j = sidx(cidx_of_k(kx, NK), iy, kz, NK, NZP);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        u0[c] = at<O32>(uh + c * plane, j);
+        q1[c] = at<O32>(K1 + c * plane, j);
+        q2[c] = at<O32>(K2 + c * plane, j);
+        q3[c] = at<O32>(K3 + c * plane, j);
+    }
+}
+
 // Epilogue of one (box, row iy, column kz) after the forward x transforms: r holds the three fields
-// at the x positions t + TPF m; es accumulates this thread's Dormand-Prince error sums.
-template <typename T, int N, int EPT, int MODE, int FAM>
+// at the x positions t + TPF m; es accumulates this thread's Dormand-Prince error sums.  SCH fixes
+// the scheme at compile time (RK4CN or IFRK4; -1: the runtime a.scheme), so that an RK4-CN
+// instance carries no exponential.
+template <typename T, int N, int EPT, int MODE, int FAM, int SCH = -1>
 __device__ __forceinline__ void p5_epilogue(const P5Args& a, const cpx_t<T> (&r)[3][EPT], int box, int iy, int kz, int t,
                                             double (&es)[3]) {
     using C = cpx_t<T>;
     constexpr int TPF = N / EPT;
     constexpr bool O32 = N <= 512;  // every component plane below 4 GiB
+    const int scheme = (SCH >= 0) ? SCH : a.scheme;
     const DevScalars& sc = a.sc[box];
     double2* __restrict__ uh = a.uh + box * a.sbox;
     double2* __restrict__ unew = a.unew ? a.unew + box * a.sbox : nullptr;  // null: update in place
@@ -1218,8 +1239,18 @@ __device__ __forceinline__ void p5_epilogue(const P5Args& a, const cpx_t<T> (&r)
     const double dt = sc.dt, nu = sc.nu;
     const long long kcut2 = (long long)kmax * kmax;
 
+    // final RK4 stage with FP32 slopes: the twelve reads of slot m's update (state and K1..K3 of the
+    // three components) are issued together before slot m's term is computed, instead of four at a
+    // time behind the previous component's store (in issue order a read behind a store waits for
+    // it); FP64 keeps the per-component reads (its batch does not fit the registers)
+    constexpr bool BATCH = MODE == EP_STAGE && FAM == 0 && sizeof(C) == 8;
+    const bool fin = BATCH && a.stage >= a.nstage;
+    double2 u0b[3];
+    C q1b[3], q2b[3], q3b[3];
+
 #pragma unroll
     for (int m = 0; m < EPT; ++m) {
+        if (fin) p5_fetch<T, N, EPT, O32>(m, t, iy, kz, kmax, NK, NZP, plane, uh, K1, K2, K3, u0b, q1b, q2b, q3b);
         const int p = t + TPF * m;
         const int kx = k_of_pos(p, N);
         if (kx < -kmax || kx > kmax || kz >= NZP) continue;
@@ -1263,7 +1294,7 @@ __device__ __forceinline__ void p5_epilogue(const P5Args& a, const cpx_t<T> (&r)
                 }
             } else {
                 double al, be;
-                stage_in_coef(a.scheme, a.stage, D, dt, al, be);
+                stage_in_coef(scheme, a.stage, D, dt, al, be);
                 const C* kp = (a.stage == 2) ? K1 : (a.stage == 3) ? K2 : K3;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
@@ -1326,7 +1357,7 @@ __device__ __forceinline__ void p5_epilogue(const P5Args& a, const cpx_t<T> (&r)
         } else {
             constexpr double B1 = 1.0 / 6.0, B2 = 1.0 / 3.0, B3 = 1.0 / 3.0, B4 = 1.0 / 6.0;
             double g, h = 0.0, e2 = 0.0;
-            if (a.scheme == RK4CN) {
+            if (scheme == RK4CN) {
                 const double Dd = D * dt;
                 g = (2.0 - Dd) / (2.0 + Dd);
                 h = 2.0 * dt / (2.0 + Dd);
@@ -1336,12 +1367,20 @@ __device__ __forceinline__ void p5_epilogue(const P5Args& a, const cpx_t<T> (&r)
             }
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const double2 u0 = at<O32>(uh + c * plane, idx);
-                const double2 q1 = to_d2<T>(at<O32>(K1 + c * plane, idx));
-                const double2 q2 = to_d2<T>(at<O32>(K2 + c * plane, idx));
-                const double2 q3 = to_d2<T>(at<O32>(K3 + c * plane, idx));
+                double2 u0, q1, q2, q3;
+                if constexpr (BATCH) {
+                    u0 = u0b[c];
+                    q1 = to_d2<T>(q1b[c]);
+                    q2 = to_d2<T>(q2b[c]);
+                    q3 = to_d2<T>(q3b[c]);
+                } else {
+                    u0 = at<O32>(uh + c * plane, idx);
+                    q1 = to_d2<T>(at<O32>(K1 + c * plane, idx));
+                    q2 = to_d2<T>(at<O32>(K2 + c * plane, idx));
+                    q3 = to_d2<T>(at<O32>(K3 + c * plane, idx));
+                }
                 double2 un;
-                if (a.scheme == RK4CN) {
+                if (scheme == RK4CN) {
                     // u (2-D)/(2+D) + 2dt/(2+D) (K1/6 + K2/3 + K3/3 + K4/6), Solver/src/solver.c:601-603
                     un.x = u0.x * g + h * (B1 * q1.x + B2 * q2.x + B3 * q3.x + B4 * nl[c].x);
                     un.y = u0.y * g + h * (B1 * q1.y + B2 * q2.y + B3 * q3.y + B4 * nl[c].y);
@@ -1403,8 +1442,9 @@ __global__ void __launch_bounds__(N / EPT * W) pass5_kernel(P5Args a) {
 // DMA (64 lanes x 16 bytes per instruction, eight x lines) while the block transforms the field
 // before, so the reads of one field overlap the transform of the previous one.  (pass5_kernel
 // holds one such block per CU, which issues all its reads, then transforms with none in flight.)
-// grid: at most one block per CU (2 x N*W elements of LDS); items = boxes * NK * (NZP / W)
-template <typename T, int N, int EPT, int W>
+// grid: at most one block per CU (2 x N*W elements of LDS); items = boxes * NK * (NZP / W).
+// SCH: the scheme (RK4CN or IFRK4), fixed at compile time.
+template <typename T, int N, int EPT, int W, int SCH>
 __global__ void __launch_bounds__(N / EPT * W) pass5p_kernel(P5Args a, int ntiles, int nitems) {
     using C = cpx_t<T>;
     constexpr int TPF = N / EPT, NT = TPF * W, NWAVE = NT / 64;
@@ -1459,7 +1499,7 @@ __global__ void __launch_bounds__(N / EPT * W) pass5p_kernel(P5Args a, int ntile
             fft<N, EPT, -1, SYNC_LDS>(r[c], t, lds, w, W, twl);
         }
         double es[3] = {0.0, 0.0, 0.0};
-        p5_epilogue<T, N, EPT, EP_STAGE, 0>(a, r, box, iy, kz, t, es);
+        p5_epilogue<T, N, EPT, EP_STAGE, 0, SCH>(a, r, box, iy, kz, t, es);
     }
 }
 
