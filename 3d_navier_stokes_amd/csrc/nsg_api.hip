@@ -41,6 +41,10 @@ struct SchedV {
     int ex64, ex32, ey64, ey32, ez64, ez32, wx64, wx32, wy64, wy32;
 };
 constexpr int nzp_of(int N) { return ((N / 3 + 1 + 15) / 16) * 16; }  // = Ext<N>::NZP
+// 1024: pass 1 takes 8 elements per thread in 4 (FP64) or 8 (FP32) columns, 64-byte row segments
+// whose two halves go to blocks launched 8 apart (xgrid); with 16 elements in 8 / 16 columns it
+// spilled 630-810 bytes per lane (measured 1024^3 step: FP64 4.18 -> 4.63, FP32-bulk 6.75 -> 8.84
+// steps/s with pass 5 as set by E5of / W5of)
 constexpr SchedV sched0(int N) {
     //                     ex64 ex32 ey64 ey32 ez64 ez32 wx64 wx32 wy64 wy32
     return N == 8     ? SchedV{8, 8, 8, 8, 4, 4, 8, 16, 8, 16}
@@ -54,7 +58,7 @@ constexpr SchedV sched0(int N) {
            : N == 128 ? SchedV{8, 8, 8, 8, 8, 8, 8, 16, 8, 16}
            : N == 256 ? SchedV{8, 8, 8, 8, 4, 8, 8, 16, 8, 16}
            : N == 512 ? SchedV{8, 8, 8, 16, 4, 8, 8, 16, 8, 16}
-                      : SchedV{16, 16, 16, 16, 8, 8, 8, 16, 8, 16};
+                      : SchedV{8, 8, 16, 16, 8, 8, 4, 8, 8, 16};
 }
 constexpr int nsched(int N) { return (N == 32 || N == 48 || N == 64 || N == 96) ? 3 : 1; }  // FFT schedules
 constexpr int kZGrid[] = {8, 4, 12};
@@ -78,9 +82,12 @@ template <typename T, int N, int S> static constexpr int WXof() { return is64<T>
 template <typename T, int N, int S> static constexpr int WYof() { return is64<T>() ? sched_of(N, S).wy64 : sched_of(N, S).wy32; }
 // pass 5 holds three fields plus the epilogue operands: at 512^3 FP32 a 1024-thread block of 8
 // elements per thread spills under its 128-VGPR cap; 512 threads of 16 elements do not (measured
-// 0.51 -> 0.48 ms per launch)
-template <typename T, int N, int S> static constexpr int E5of() { return (!is64<T>() && N == 512) ? 16 : EXof<T, N, S>(); }
-template <typename T, int N, int S> static constexpr int W5of() { return WXof<T, N, S>(); }
+// 0.51 -> 0.48 ms per launch).  At 1024^3 FP64 pass 5 keeps 16 elements in 8 columns: it spills, but
+// ran 7.29 ms per launch against 7.69 ms for 8 elements in 4 columns (no spill).
+template <typename T, int N, int S> static constexpr int E5of() {
+    return ((!is64<T>() && N >= 512) || (is64<T>() && N == 1024)) ? 16 : EXof<T, N, S>();
+}
+template <typename T, int N, int S> static constexpr int W5of() { return (is64<T>() && N == 1024) ? 8 : WXof<T, N, S>(); }
 // pass 5 with its field reads pipelined through LDS (pass5p_kernel) for the RK4-family stages at
 // 512^3: measured per launch FP32 0.334 -> 0.277 ms (stages 1-3) and 0.903 -> 0.865 ms (stage 4),
 // FP64 0.529 -> 0.494 and 1.296 -> 1.282 ms
@@ -188,7 +195,10 @@ template <typename T> static const void* twM_of(nsg_ctx* c) {
 template <typename T> static constexpr size_t esz() { return sizeof(cpx_t<T>); }
 
 // grid of the x passes: (tiles, rows, boxes), or XCD-paired tiles when a tile row is under 128 B
+// x-pass grid; below 128 bytes per row segment, tiles go in pairs (ntiles = NZP / W is even: NZP is
+// a multiple of 16 and W <= 8 there)
 template <typename T, int W> static dim3 xgrid(int ntiles, int nrows, int nb) {
+    static_assert(W * sizeof(cpx_t<T>) >= 128 || 16 % W == 0, "paired tiles need an even tile count");
     if (W * sizeof(cpx_t<T>) >= 128) return dim3(ntiles, nrows, nb);
     const long long pairs = (long long)(ntiles / 2) * nrows;
     return dim3((unsigned)(16 * ((pairs + 7) / 8)), 1, nb);

@@ -86,6 +86,25 @@ def test_fp32bulk_vs_fp64_512(nsg):
     assert abs(tb.dissipation - ta.dissipation) < 1e-6 * ta.dissipation
 
 
+def test_fp32bulk_vs_fp64_1024(nsg):
+    """1024^3, where pass 1 takes 64-byte row segments in pairs of blocks and the two precisions use
+    different x-pass layouts: one step from the same k^4 state agrees in E(k) to 1e-5 on the shells
+    holding at least 1e-10 of the peak shell's energy, and in energy and dissipation"""
+    N = 1024
+    kw = dict(nu=1e-3, dt=1e-4, force_eps=0.1)
+    res = []
+    for prec in ("fp64", "fp32bulk"):
+        with nsg.Solver(N, precision=prec, **kw) as s:
+            s.init("k4", k0=4.0, energy=0.5, seed=11)
+            s.step(1)
+            res.append(s.measures())
+    (ta, ea, _), (tb, eb, _) = res
+    sel = ea >= 1e-10 * ea.max()
+    assert np.max(np.abs(eb[sel] - ea[sel]) / ea[sel]) < 1e-5
+    assert abs(tb.energy - ta.energy) < 1e-7 * ta.energy
+    assert abs(tb.dissipation - ta.dissipation) < 1e-6 * ta.dissipation
+
+
 def test_determinism_512(nsg):
     """two runs from the same state are bitwise identical (no order-dependent reductions)"""
     N = 512
