@@ -88,6 +88,13 @@ template <typename T, int N, int S> static constexpr int E5of() {
     return ((!is64<T>() && N >= 512) || (is64<T>() && N == 1024)) ? 16 : EXof<T, N, S>();
 }
 template <typename T, int N, int S> static constexpr int W5of() { return (is64<T>() && N == 1024) ? 8 : WXof<T, N, S>(); }
+// pass 5 of the Dormand-Prince family (RK5-CN, DP5: the final update reads five or six slopes): at
+// 512^3 FP32 takes 8 elements in 8 columns (512 threads, 180 VGPRs, paired 64-byte row segments;
+// the RK4 layout spilled 184 bytes per lane): RK5-CN 52.2 -> 53.1 steps/s.  FP64 keeps the RK4
+// layout, which spills 84 bytes but ran faster than the spill-free 8 elements in 4 columns (RK5-CN
+// 30.5 vs 29.6 steps/s)
+template <typename T, int N, int S> static constexpr int E5dp() { return (N == 512 && !is64<T>()) ? 8 : E5of<T, N, S>(); }
+template <typename T, int N, int S> static constexpr int W5dp() { return (N == 512 && !is64<T>()) ? 8 : W5of<T, N, S>(); }
 // pass 5 with its field reads pipelined through LDS (pass5p_kernel) for the RK4-family stages at
 // 512^3: measured per launch FP32 0.334 -> 0.277 ms (stages 1-3) and 0.903 -> 0.865 ms (stage 4),
 // FP64 0.529 -> 0.494 and 1.296 -> 1.282 ms
@@ -359,10 +366,12 @@ static int launch_passz(nsg_ctx* c, const void* in, void* out, long long ibox, l
     return NSG_OK;
 }
 
-template <typename T, int N, int S, int MODE>
+// DPF: a Dormand-Prince family context (pass5_kernel FAM 1 in its own layout)
+template <typename T, int N, int S, int MODE, bool DPF = false>
 static int launch_pass5(nsg_ctx* c, double2* uh, int b0, int nb, int stage, void* kout, int force, double norm,
                         double2* unew = nullptr, double* errpart = nullptr) {
-    constexpr int EPT = E5of<T, N, S>(), W = W5of<T, N, S>(), TPF = N / EPT;
+    constexpr int EPT = DPF ? E5dp<T, N, S>() : E5of<T, N, S>(), W = DPF ? W5dp<T, N, S>() : W5of<T, N, S>();
+    constexpr int TPF = N / EPT;
     static_assert(TPF * W <= 1024, "block too large");
     constexpr size_t e = esz<T>();
     P5Args a;
@@ -382,8 +391,8 @@ static int launch_pass5(nsg_ctx* c, double2* uh, int b0, int nb, int stage, void
     a.kmax = c->kmax; a.NK = c->NK; a.NZP = c->NZP;
     const dim3 grid = xgrid<T, W>(c->NZP / W, c->NK, nb);
     prof_begin(c);
-    if constexpr (MODE == EP_STAGE && p5_pipelined<T, N, W>()) {
-        if (!dp_family(c)) {
+    if constexpr (MODE == EP_STAGE && !DPF && p5_pipelined<T, N, W>()) {
+        {
             const int ntiles = c->NZP / W;
             const long long nitems = (long long)ntiles * c->NK * nb;
             const int blocks = (int)std::min<long long>(nitems, c->ncu);  // 2 N W elements of LDS: one block per CU
@@ -399,7 +408,7 @@ static int launch_pass5(nsg_ctx* c, double2* uh, int b0, int nb, int stage, void
         }
     }
     if constexpr (MODE == EP_STAGE) {
-        if (dp_family(c))
+        if constexpr (DPF)
             hipLaunchKernelGGL((pass5_kernel<T, N, EPT, W, MODE, 1>), grid, dim3(TPF * W), 0, c->s, a);
         else
             hipLaunchKernelGGL((pass5_kernel<T, N, EPT, W, MODE, 0>), grid, dim3(TPF * W), 0, c->s, a);
@@ -436,8 +445,11 @@ static int run_stage(nsg_ctx* c, double2* uh, int b0, int nb, int stage, bool tr
     if ((rc = launch_passy<T, N, S, -1>(c, b0, nb, 3))) return rc;
     void* kout = final_update ? nullptr : c->K[stage - 1];
     const bool dp = final_update && c->p.scheme == NSG_SCHEME_DP5;
+    if (dp_family(c))
+        return launch_pass5<T, N, S, EP_STAGE, true>(c, uh, b0, nb, final_update ? c->nstage : stage, kout, force,
+                                                     1.0 / (N3 * N3), dp ? c->ucand : nullptr, dp ? c->dperr : nullptr);
     return launch_pass5<T, N, S, EP_STAGE>(c, uh, b0, nb, final_update ? c->nstage : stage, kout, force,
-                                           1.0 / (N3 * N3), dp ? c->ucand : nullptr, dp ? c->dperr : nullptr);
+                                           1.0 / (N3 * N3), nullptr, nullptr);
 }
 
 // one step of boxes [b0, b0+nb); Dormand-Prince: one try, whose candidate lands in ucand and
@@ -449,7 +461,7 @@ template <typename T, int N, int S> static int run_step(nsg_ctx* c, int b0, int 
     for (int s = 1; s <= c->nstage; ++s)
         if ((rc = run_stage<T, N, S>(c, c->uh, b0, nb, s, s == 1, s == c->nstage))) return rc;
     if (c->p.scheme == NSG_SCHEME_DP5) {
-        constexpr int W = W5of<T, N, S>();
+        constexpr int W = W5dp<T, N, S>();
         const int nslots = c->NK * (c->NZP / W);
         const double count = (double)N * N * (N / 2 + 1);
         hipLaunchKernelGGL(dp_finalize_kernel, dim3(nb), dim3(256), 0, c->s, c->sc + b0,

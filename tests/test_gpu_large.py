@@ -105,6 +105,46 @@ def test_fp32bulk_vs_fp64_1024(nsg):
     assert abs(tb.dissipation - ta.dissipation) < 1e-6 * ta.dissipation
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32bulk", 1e-6)])
+def test_rk5cn_beltrami_decay_512(nsg, precision, tol):
+    """the 512^3 Dormand-Prince-family pass 5 on the Beltrami field:
+    NL = 0, so every RK5-CN step multiplies the state by (2 - D)/(2 + D)"""
+    N, nu, dt = 512, 1e-3, 1e-3
+    with nsg.Solver(N, nu=nu, dt=dt, scheme="rk5cn", precision=precision) as s:
+        s.init("shapiro")
+        u0 = s.get_state()
+        s.step(2)
+        got = s.get_state()
+    D = nu * 12 * dt
+    assert rel(got, u0 * ((2 - D) / (2 + D)) ** 2) < tol
+
+
+def test_dormand_prince_pass5_512(nsg):
+    """the 512^3 Dormand-Prince-family pass 5 (FP32 in paired 64-byte row segments, FP64 in whole
+    ones) from the same k^4 state: one RK5-CN step in FP32-bulk agrees with FP64 in E(k) to 1e-5 on the shells
+    above the FP32 noise floor; one DP5 step (FP64, accepted at once) lands on the RK5-CN state
+    (same 5th-order weights), and its error norm, reduced from the per-block sums, sets the next dt"""
+    N = 512
+    kw = dict(nu=1e-3, dt=1e-3)
+    res = {}
+    for scheme, prec in (("rk5cn", "fp64"), ("rk5cn", "fp32bulk"), ("dp5", "fp64")):
+        with nsg.Solver(N, precision=prec, scheme=scheme, **kw) as s:
+            s.init("k4", k0=2.0, energy=0.5, seed=2)
+            s.step(1)
+            res[scheme, prec] = (s.measures(), s.time(), s.dp_stats() if scheme == "dp5" else None)
+    (ta, ea, _), _, _ = res["rk5cn", "fp64"]
+    (tb, eb, _), _, _ = res["rk5cn", "fp32bulk"]
+    (tc, ec, _), (t, dtn, _), (acc, rej, err) = res["dp5", "fp64"]
+    sel = ea >= 1e-10 * ea.max()
+    assert np.max(np.abs(eb[sel] - ea[sel]) / ea[sel]) < 1e-5
+    assert abs(tb.energy - ta.energy) < 1e-7 * ta.energy
+    assert acc == 1 and rej == 0 and t == 1e-3
+    assert abs(tc.energy - ta.energy) < 1e-12 * ta.energy
+    assert np.max(np.abs(ec[sel] - ea[sel]) / ea[sel]) < 1e-10
+    assert 0.0 < err <= 1.0
+    assert abs(dtn - 1e-3 * min(1.5, max(0.01, 0.8 * (1.0 / err) ** 0.2))) < 1e-14
+
+
 def test_determinism_512(nsg):
     """two runs from the same state are bitwise identical (no order-dependent reductions)"""
     N = 512
