@@ -135,6 +135,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.nsg_attach_closure.argtypes = [vp, vp]
     lib.nsg_resample.argtypes = [vp, vp]
     lib.nsg_closure_term.argtypes = [vp, vp]
+    lib.nsg_closure_set_gain.argtypes = [vp, vp, ctypes.c_int]
     # increment statistics
     lib.nsg_incr_create.argtypes = [ctypes.c_int, ctypes.c_int, P(vp)]
     lib.nsg_incr_destroy.argtypes = [vp]
@@ -334,6 +335,15 @@ class Solver:
         out = np.empty(self.spectral_shape, dtype=np.complex128)
         self._check(self.lib.nsg_closure_term(self._h, _ptr(out)))
         return out
+
+    def set_closure_gain(self, gain=None):
+        """Multiply the attached closure term on the shell b = round(|k|) by gain[b], b = 0..N//3
+        (None removes the gains)."""
+        if gain is None:
+            self._check(self.lib.nsg_closure_set_gain(self._h, None, 0))
+            return
+        g = np.ascontiguousarray(gain, dtype=np.float64)
+        self._check(self.lib.nsg_closure_set_gain(self._h, _ptr(g), int(g.size)))
 
     def autotune(self, iters: int = 3, use_cache: bool = True):
         """Time every launch schedule of this size on the current state (restored afterwards) and

@@ -73,9 +73,11 @@ __global__ void unpack_tau_kernel(const float* __restrict__ o, long long n, doub
     }
 }
 
-// M_i = -i sum_j k_j tau_ij, projected (k = 0 -> 0), 2/3 mask.  Sd = (xx, yy, zz), So = (xy, xz, yz)
+// M_i = -i sum_j k_j tau_ij, projected (k = 0 -> 0), 2/3 mask.  Sd = (xx, yy, zz), So = (xy, xz, yz).
+// gain (or null): M times gain[b] on the shell b = round(|k|) (k^2 in [b^2 - b + 1, b^2 + b], the
+// shells of the energy spectrum)
 __global__ void closure_combine_kernel(const double2* __restrict__ Sd, const double2* __restrict__ So, double2* __restrict__ M,
-                                       int kmax, int NK, int NZP) {
+                                       int kmax, int NK, int NZP, const double* __restrict__ gain) {
     const long long plane = (long long)NK * NK * NZP;
     const long long kcut2 = (long long)kmax * kmax;
     for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < plane; n += (long long)gridDim.x * blockDim.x) {
@@ -97,6 +99,15 @@ __global__ void closure_combine_kernel(const double2* __restrict__ Sd, const dou
             m[0].x -= a * pr; m[0].y -= a * pi;
             m[1].x -= b * pr; m[1].y -= b * pi;
             m[2].x -= c * pr; m[2].y -= c * pi;
+            if (gain) {
+                long long b = (long long)sqrt((double)k2);
+                while (b * b > k2) --b;
+                while ((b + 1) * (b + 1) <= k2) ++b;
+                if (k2 > b * b + b) ++b;  // round(|k|)
+                const double g = gain[b];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) m[i] = make_double2(g * m[i].x, g * m[i].y);
+            }
         }
 #pragma unroll
         for (int i = 0; i < 3; ++i) M[i * plane + n] = m[i];
@@ -191,9 +202,11 @@ void nsgi_closure_free(nsg_ctx* c) {
     if (c->closure) (void)hipFree(c->closure);
     if (c->cl_real) (void)hipFree(c->cl_real);
     if (c->cl_spec) (void)hipFree(c->cl_spec);
+    if (c->cl_gain) (void)hipFree(c->cl_gain);
     c->closure = nullptr;
     c->cl_real = nullptr;
     c->cl_spec = nullptr;
+    c->cl_gain = nullptr;
     c->cnn = nullptr;
 }
 
@@ -217,7 +230,7 @@ int nsgi_closure_eval(nsg_ctx* c) {
     if ((rc = nsgi_r2c(c, Td, S0))) return rc;
     if ((rc = nsgi_r2c(c, To, S1))) return rc;
     hipLaunchKernelGGL(closure_combine_kernel, dim3(GB), dim3(TB), 0, c->s, (const double2*)S0, (const double2*)S1, c->closure,
-                       c->kmax, c->NK, c->NZP);
+                       c->kmax, c->NK, c->NZP, (const double*)c->cl_gain);
     LAUNCHCHK(c);
     return NSG_OK;
 }
@@ -239,6 +252,24 @@ int nsg_attach_closure(nsg_ctx* c, nsg_cnn* net) {
     int rc = alloc_closure(c);
     if (rc) return rc;
     c->cnn = net;
+    return NSG_OK;
+}
+
+int nsg_closure_set_gain(nsg_ctx* c, const double* gain, int n) {
+    if (!c) return NSG_ERR_INVALID;
+    if (!c->cnn) return fail(c, NSG_ERR_INVALID, "no closure attached");
+    HIPCHK(c, hipSetDevice(c->p.device));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    if (!gain) {
+        if (c->cl_gain) (void)hipFree(c->cl_gain);
+        c->cl_gain = nullptr;
+        return NSG_OK;
+    }
+    if (n != c->kmax + 1) return fail(c, NSG_ERR_INVALID, "closure gains: %d shells given, %d expected (0..kmax)", n, c->kmax + 1);
+    for (int b = 0; b < n; ++b)
+        if (!std::isfinite(gain[b])) return fail(c, NSG_ERR_INVALID, "closure gain of shell %d is not finite", b);
+    if (!c->cl_gain) HIPCHK(c, hipMalloc(&c->cl_gain, sizeof(double) * n));
+    HIPCHK(c, hipMemcpy(c->cl_gain, gain, sizeof(double) * n, hipMemcpyHostToDevice));
     return NSG_OK;
 }
 

@@ -54,6 +54,7 @@ struct nsg_ctx {
     double2* closure = nullptr;
     double* cl_real = nullptr;    // 5 real AoS fields [N^3][3]: 3 gradient groups, tau diagonal, tau off-diagonal
     double2* cl_spec = nullptr;   // 2 compact FP64 3-vectors
+    double* cl_gain = nullptr;    // per-shell gains of the closure term (kmax + 1 shells), or null
 };
 
 static int fail(nsg_ctx* c, int code, const char* fmt, ...) __attribute__((format(printf, 3, 4), unused));

@@ -9,8 +9,10 @@
 2. Training of the closure network on those samples (BF16 MFMA forward/backward, Adam), the last
    samples held out.  The model's mean subgrid dissipation -<tau_ij S_ij> is compared with the
    true one on the training samples; their ratio is the gain of a calibrated copy of the model.
+   Per shell of |k|, the ratio of the true to the model's subgrid transfer spectrum on the training
+   samples is a second, per-shell calibration (applied to the closure term by the library).
 3. LES runs on the coarse grid from the last filtered DNS state, same forcing and viscosity: with
-   the trained closure, with its calibrated copy, without a model (the plain coarse DNS, the
+   the trained closure, with its calibrated copy, with its per-shell calibration, without a model (the plain coarse DNS, the
    fallback), and without a model at the constant eddy viscosity
    nu_eff = eps_in (2 pi)^3 / (2 Omega_f) that balances the injection with the filtered field's
    enstrophy Omega_f (a volume integral, like every measure here).  Each run's time-averaged E(k)
@@ -145,6 +147,30 @@ def rotate_sample(x, tau, N, R):
     return g.reshape(-1, 9).astype(np.float32), t_out.astype(np.float32)
 
 
+def sgs_transfer(x, tau, N):
+    """per-shell subgrid energy transfer T(b) = sum over the modes of the shell b = round(|k|) of
+    Re(conj(S_ij) tau_ij) (half spectrum, the kz = 0 and N/2 planes counted once): the energy the
+    term -P div(tau) moves into the resolved modes of that shell is -T(b) up to normalisation, since
+    conj(u_i) (-i k_j tau_ij) = conj(i k_j u_i) tau_ij and tau is symmetric.  x [N^3, 9], tau [N^3, 6]
+    (xx, yy, zz, xy, xz, yz); returns b = 0..N // 3."""
+    g = x.astype(np.float64).reshape(N, N, N, 3, 3)
+    S = 0.5 * (g + np.swapaxes(g, -1, -2))
+    t = tau.astype(np.float64).reshape(N, N, N, 6)
+    pairs = [(0, 0, 0, 1.0), (1, 1, 1, 1.0), (2, 2, 2, 1.0), (0, 1, 3, 2.0), (0, 2, 4, 2.0), (1, 2, 5, 2.0)]
+    acc = 0.0
+    for i, j, c, w in pairs:
+        acc = acc + w * np.real(np.conj(np.fft.rfftn(S[..., i, j])) * np.fft.rfftn(t[..., c]))
+    k = np.fft.fftfreq(N, 1.0 / N).astype(np.int64)
+    kz = np.arange(N // 2 + 1, dtype=np.int64)
+    k2 = k[:, None, None] ** 2 + k[None, :, None] ** 2 + kz[None, None, :] ** 2
+    b = np.floor(np.sqrt(k2)).astype(np.int64)
+    b += k2 > b * b + b  # round(|k|): k^2 in [b^2 - b + 1, b^2 + b]
+    wz = np.where((kz == 0) | (kz == N // 2), 1.0, 2.0)[None, None, :]
+    kmax = N // 3
+    sel = b <= kmax
+    return np.bincount(b[sel], weights=(acc * wz)[sel], minlength=kmax + 1)[: kmax + 1]
+
+
 def train(nsg, samples, Nl, steps, batch, lr, holdout, seed=1, augment=0):
     """train on samples[:-holdout]; `augment` extra copies of each training sample in random cube
     rotations (the closure has no preferred axis)"""
@@ -195,13 +221,25 @@ def train(nsg, samples, Nl, steps, batch, lr, holdout, seed=1, augment=0):
            "loss_history_every_50": hist[::50]}
     # SGS dissipation of the model against the truth on the training samples; the gain that
     # matches them calibrates the output (a-priori, training data only)
+    # and per shell, the subgrid transfer spectra of the two: their ratio (clipped to [1/2, 2], 1 where
+    # the model's transfer is negligible) is the per-shell gain of a second calibration
     pi_true, pi_model = 0.0, 0.0
+    tr_true, tr_model = 0.0, 0.0
     for x, t in samples[:ntrain]:
         y = net.forward((x / sig_in)[None])[0] * np.array(sig_out)
         pi_true += sgs_dissipation(x, t)
         pi_model += sgs_dissipation(x, y)
+        tr_true = tr_true + sgs_transfer(x, t, Nl)
+        tr_model = tr_model + sgs_transfer(x, y, Nl)
+    ok = np.abs(tr_model) > 1e-6 * np.max(np.abs(tr_model))
+    shell_gain = np.where(ok, tr_true / np.where(ok, tr_model, 1.0), 1.0)
+    shell_gain = np.clip(shell_gain, 0.5, 2.0)
+    shell_gain[0] = 1.0
     res.update({"sgs_dissipation_true": pi_true / ntrain, "sgs_dissipation_model": pi_model / ntrain,
-                "dissipation_gain": pi_true / pi_model if pi_model > 0 else 1.0})
+                "dissipation_gain": pi_true / pi_model if pi_model > 0 else 1.0,
+                "sgs_transfer_true_by_shell": (tr_true / ntrain).tolist(),
+                "sgs_transfer_model_by_shell": (tr_model / ntrain).tolist(),
+                "shell_gain": shell_gain.tolist()})
     return net, res
 
 
@@ -217,11 +255,13 @@ def scaled_network(nsg, net, gain, Nl, batch):
     return out
 
 
-def run_les(nsg, Nl, ub, nu, steps, net=None, avg_from=0.2, eps=0.1):
+def run_les(nsg, Nl, ub, nu, steps, net=None, avg_from=0.2, eps=0.1, gain=None):
     s = forced(nsg, Nl, nu, "fp64")
     s.set_state(ub)
     if net is not None:
         s.attach_closure(net)
+        if gain is not None:
+            s.set_closure_gain(gain)
     spectra, t0w = [], time.perf_counter()
     t_start = s.time()[0]
     blow = None
@@ -282,9 +322,11 @@ def main():
     nu_eff = 0.1 * (2.0 * np.pi) ** 3 / (2.0 * dstats["filtered_enstrophy"])
     cal = scaled_network(nsg, net, tstats["dissipation_gain"], a.les, a.batch)
     runs = {}
-    for name, net_, nu_ in [("cnn_closure", net, a.nu), ("cnn_closure_dissipation_calibrated", cal, a.nu),
-                            ("no_model", None, a.nu), ("eddy_viscosity", None, nu_eff)]:
-        r = run_les(nsg, a.les, ub, nu_, a.les_steps, net_)
+    gain = np.array(tstats["shell_gain"])
+    for name, net_, nu_, g_ in [("cnn_closure", net, a.nu, None), ("cnn_closure_dissipation_calibrated", cal, a.nu, None),
+                                ("cnn_closure_shell_calibrated", net, a.nu, gain),
+                                ("no_model", None, a.nu, None), ("eddy_viscosity", None, nu_eff, None)]:
+        r = run_les(nsg, a.les, ub, nu_, a.les_steps, net_, gain=g_)
         emax, emean = spectral_error(r["E_k"], ek_ref, kc)
         r.update({"nu": nu_, "E_k_max_rel_err_k_le_kc": emax, "E_k_mean_rel_err_k_le_kc": emean})
         log(f"[les] {name}: max rel E(k) err {emax}, mean {emean}, {r['ms_per_step']:.3f} ms/step, blowup {r['blowup']}")

@@ -226,6 +226,10 @@ int nsg_sgs_sample(nsg_ctx* dns, nsg_ctx* les, double kc, float* x_out, float* t
 int nsg_resample(nsg_ctx* src, nsg_ctx* dst);
 int nsg_attach_closure(nsg_ctx* ctx, nsg_cnn* net);
 int nsg_closure_term(nsg_ctx* ctx, double* out);
+/* per-shell gains of the attached closure term (no reference counterpart): the term's modes on the
+ * shell b = round(|k|) (the energy spectrum's shells) are multiplied by gain[b], b = 0..kmax
+ * (n = kmax + 1); NULL removes them.  They stay until removed or the closure is detached. */
+int nsg_closure_set_gain(nsg_ctx* ctx, const double* gain, int n);
 
 /* ---------------------------------------------------------------------------------------------
  * Velocity-increment statistics (the reference's post-processing, PostProcessing/src/stats.c).

@@ -128,6 +128,33 @@ def test_closure_term_vs_oracle(nsg, oracle, N):
     assert rel(M, want) < 1e-2
 
 
+def test_closure_gain_per_shell(nsg, oracle):
+    """per-shell gains scale the closure term's modes on the shell round(|k|) and nothing else; no
+    gains (removed) give the plain term bit for bit; a wrong shell count is refused"""
+    from conftest import wavenumbers
+    N = 32
+    uh = oracle.ic_k4(N, 3.0, 0.5, 29)
+    with nsg.Network(N, seed=8) as net, nsg.Solver(N, nu=1e-3, dt=1e-3) as les:
+        net.set_norm(2.5, [0.05, 0.05, 0.05, 0.03, 0.03, 0.03])
+        les.set_state(uh)
+        les.attach_closure(net)
+        m0 = les.closure_term()
+        gain = 0.5 + np.random.default_rng(3).random(N // 3 + 1)
+        les.set_closure_gain(gain)
+        m1 = les.closure_term()
+        les.set_closure_gain(None)
+        m2 = les.closure_term()
+        with pytest.raises(nsg.NSGError):
+            les.set_closure_gain(gain[:-1])
+    KX, KY, KZ = wavenumbers(N)
+    k2 = (KX * KX + KY * KY + KZ * KZ).astype(np.int64)
+    b = np.floor(np.sqrt(k2)).astype(np.int64)
+    b += (k2 > b * b + b)  # round(|k|): k^2 in [b^2 - b + 1, b^2 + b]
+    g = np.where(b <= N // 3, gain[np.minimum(b, N // 3)], 0.0)
+    assert np.max(np.abs(m1 - g[..., None] * m0)) <= 1e-15 * np.max(np.abs(m0))
+    assert np.array_equal(m2, m0)
+
+
 @pytest.mark.parametrize("N", [32, 64])
 def test_les_step_with_closure(nsg, oracle, N):
     """zero network: the step is bitwise the plain step; trained-free random network: the state
