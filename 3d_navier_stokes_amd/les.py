@@ -53,8 +53,9 @@ def forced(nsg, N, nu, precision, eps=0.1, kf=2.5, cfl=0.5):
     return nsg.Solver(N, nu=nu, dt=1e-3, precision=precision, cfl=cfl, force_eps=eps, force_kf=kf)
 
 
-def generate(nsg, Nd, Nl, kc, nu, spin_n, spin_time, settle, nsamples, every, seed=2, precision="fp32bulk"):
-    """forced DNS -> (samples [(x, tau)], mean filtered E(k), last filtered state, stats).  The
+def generate(nsg, Nd, Nl, kc, nu, spin_n, spin_time, settle, nsamples, every, seed=2, precision="fp32bulk", starts=1):
+    """forced DNS -> (samples [(x, tau)], mean filtered E(k), filtered states to start LES runs from
+    (`starts` of them, evenly spread over the window, the last one at its end), stats).  The
     statistically steady state is reached cheaply: the run starts on a spin_n^3 grid until
     t >= spin_time, is resampled (zero-padded) onto the Nd^3 grid and settles for `settle` steps
     before sampling starts."""
@@ -74,7 +75,7 @@ def generate(nsg, Nd, Nl, kc, nu, spin_n, spin_time, settle, nsamples, every, se
     t_spin = time.perf_counter() - t0
     log(f"[les] spin-up {nspin} steps at {spin_n}^3 + {settle} at {Nd}^3: {t_spin:.1f} s, t = {dns.time()[0]:.3f}")
     les = nsg.Solver(Nl, nu=nu)
-    samples, spectra, enst, energy = [], [], [], []
+    samples, spectra, enst, energy, ubs = [], [], [], [], []
     t_dns, n_dns, t_phys0 = 0.0, 0, dns.time()[0]
     for s in range(nsamples):
         t0 = time.perf_counter()
@@ -84,13 +85,14 @@ def generate(nsg, Nd, Nl, kc, nu, spin_n, spin_time, settle, nsamples, every, se
         x, tau = dns.sgs_sample(les, kc)
         tot, ek, _ = les.measures()
         samples.append((x, tau))
+        if (nsamples - 1 - s) % max(nsamples // starts, 1) == 0 and len(ubs) < starts:
+            ubs.append(les.get_state())
         spectra.append(ek)
         enst.append(tot.enstrophy)
         energy.append(dns.measures()[0].energy)
         if (s + 1) % 16 == 0:
             log(f"[les] sample {s + 1}/{nsamples}, t = {dns.time()[0]:.3f}")
     t_phys = dns.time()[0] - t_phys0
-    ub = les.get_state()
     h = len(spectra) // 2
     e1, e2 = np.mean(spectra[:h], axis=0), np.mean(spectra[h:], axis=0)
     kk = np.arange(1, int(kc) + 1)
@@ -101,7 +103,7 @@ def generate(nsg, Nd, Nl, kc, nu, spin_n, spin_time, settle, nsamples, every, se
              "filtered_enstrophy": float(np.mean(enst)), "dns_energy_first_last": [energy[0], energy[-1]]}
     dns.close()
     les.close()
-    return samples, np.mean(spectra, axis=0), ub, stats
+    return samples, np.mean(spectra, axis=0), ubs, stats
 
 
 def sgs_dissipation(x, tau):
@@ -308,12 +310,14 @@ def main():
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--les-steps", type=int, default=5000)
+    ap.add_argument("--les-starts", type=int, default=1,
+                    help="LES runs per variant, from filtered states spread over the window (E(k) averaged)")
     ap.add_argument("--out", default=None)
     a = ap.parse_args()
     kc = a.kc if a.kc else a.les // 3
     nsg = load_nsg()
-    samples, ek_ref, ub, dstats = generate(nsg, a.dns, a.les, kc, a.nu, a.spin_n, a.spin_time, a.settle, a.samples,
-                                           a.every)
+    samples, ek_ref, ubs, dstats = generate(nsg, a.dns, a.les, kc, a.nu, a.spin_n, a.spin_time, a.settle, a.samples,
+                                            a.every, starts=a.les_starts)
     log(f"[les] {len(samples)} samples; DNS {dstats['dns_ms_per_step']:.2f} ms/step")
     net, tstats = train(nsg, samples, a.les, a.train_steps, a.batch, a.lr, a.holdout, augment=a.augment)
     log(f"[les] training: loss {tstats['loss_train_initial']:.4f} -> {tstats['loss_train_final']:.4f}, holdout "
@@ -326,9 +330,16 @@ def main():
     for name, net_, nu_, g_ in [("cnn_closure", net, a.nu, None), ("cnn_closure_dissipation_calibrated", cal, a.nu, None),
                                 ("cnn_closure_shell_calibrated", net, a.nu, gain),
                                 ("no_model", None, a.nu, None), ("eddy_viscosity", None, nu_eff, None)]:
-        r = run_les(nsg, a.les, ub, nu_, a.les_steps, net_, gain=g_)
+        per = [run_les(nsg, a.les, ub, nu_, a.les_steps, net_, gain=g_) for ub in ubs]
+        r = dict(per[-1])
+        eks = [p["E_k"] for p in per if p["E_k"] is not None]
+        r["E_k"] = np.mean(eks, axis=0) if eks else None
+        if len(per) > 1:
+            r["E_k_per_start"] = [p["E_k"].tolist() if p["E_k"] is not None else None for p in per]
+            r["E_k_max_rel_err_per_start"] = [spectral_error(p["E_k"], ek_ref, kc)[0] for p in per]
+            r["blowup"] = [p["blowup"] for p in per]
         emax, emean = spectral_error(r["E_k"], ek_ref, kc)
-        r.update({"nu": nu_, "E_k_max_rel_err_k_le_kc": emax, "E_k_mean_rel_err_k_le_kc": emean})
+        r.update({"nu": nu_, "E_k_max_rel_err_k_le_kc": emax, "E_k_mean_rel_err_k_le_kc": emean, "starts": len(per)})
         log(f"[les] {name}: max rel E(k) err {emax}, mean {emean}, {r['ms_per_step']:.3f} ms/step, blowup {r['blowup']}")
         runs[name] = r
     les_s_per_tu = runs["cnn_closure"]["gpu_s"] / max(runs["cnn_closure"]["time_units"], 1e-30)
