@@ -196,8 +196,9 @@ def test_trained_closure_beats_no_model(nsg):
     L = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(L)
     Nd, Nl, kc, nu = 128, 32, 10.0, 5e-3
-    samples, ek_ref, ub, _ = L.generate(nsg, Nd, Nl, kc, nu, spin_n=64, spin_time=8.0, settle=300, nsamples=24,
-                                        every=40)
+    samples, ek_ref, ubs, _ = L.generate(nsg, Nd, Nl, kc, nu, spin_n=64, spin_time=8.0, settle=300, nsamples=24,
+                                         every=40)
+    ub = ubs[-1]  # the filtered state at the end of the window
     net, st = L.train(nsg, samples, Nl, steps=600, batch=2, lr=1e-3, holdout=4, augment=1)
     assert st["loss_holdout_final"] < st["loss_holdout_initial"]
     errs = {}
