@@ -42,7 +42,6 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
     u += 0x7fffu + ((u >> 16) & 1u);
     return (uint16_t)(u >> 16);
 }
-__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 // the same rounding (to nearest, ties to even) by gfx950's conversion instruction
 __device__ __forceinline__ uint16_t f2bf_hw(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
@@ -124,9 +123,11 @@ struct ConvArgs {
 // grid (tiles, 1, batch); block 512.  The layer's weights are staged in LDS next to the input
 // halo (row stride 27 CIN + 8 halfwords: the 16 rows of one B fragment fall in distinct banks);
 // the next K step's fragments are loaded while the current step's MFMAs run; bf16 outputs leave
-// through an LDS tile as 16-byte stores.
+// through an LDS tile as 16-byte stores.  Used for the input gradient (EPI_BWD_GELU: flipped,
+// transposed weights, GELU' of the stored pre-activation); the forward modes run in conv3p_kernel.
 template <int CIN, int COUT, int EPI>
 __global__ void __launch_bounds__(CONV_THREADS) conv3_kernel(ConvArgs a) {
+    static_assert(EPI == EPI_BWD_GELU, "input gradients only");
     constexpr int SP = CIN + 8;
     constexpr int KT = 27 * CIN;
     constexpr int WS = KT + 8;
@@ -191,7 +192,7 @@ __global__ void __launch_bounds__(CONV_THREADS) conv3_kernel(ConvArgs a) {
         }
     }
     // epilogue: lane holds rows (points) 4g..4g+3 of column (channel) m of each block
-    if constexpr (EPI != EPI_LINEAR) __syncthreads();  // the halo buffer becomes the output tile
+    __syncthreads();  // the halo buffer becomes the output tile
 #pragma unroll
     for (int s = 0; s < SUB; ++s) {
         const int st = wv * SUB + s, ix = st / TYT, iy = st % TYT;
@@ -199,34 +200,21 @@ __global__ void __launch_bounds__(CONV_THREADS) conv3_kernel(ConvArgs a) {
 #pragma unroll
         for (int h = 0; h < NH; ++h) {
             const int co = h * 16 + m;
-            const float bias = (EPI == EPI_BWD_GELU) ? 0.f : a.bias[co];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const long long idx = (row + 4 * g + i) * COUT + co;
                 const int tp = st * TZT + 4 * g + i;  // point index within the tile (z fastest)
-                const float v = acc[s][h][i] + bias;
-                if constexpr (EPI == EPI_GELU) {
-                    lds[tp * OS + co] = f2bf(gelu(v));
-                } else if constexpr (EPI == EPI_GELU_PRE) {
-                    a.f_out[idx] = v;
-                    lds[tp * OS + co] = f2bf(gelu(v));
-                } else if constexpr (EPI == EPI_LINEAR) {
-                    a.f_out[idx] = v;
-                } else {
-                    lds[tp * OS + co] = f2bf(v * gelu_grad(a.pre_in[idx]));
-                }
+                lds[tp * OS + co] = f2bf(acc[s][h][i] * gelu_grad(a.pre_in[idx]));
             }
         }
     }
-    if constexpr (EPI != EPI_LINEAR) {
-        __syncthreads();
-        constexpr int CH = COUT / 8;
-        for (int i = threadIdx.x; i < TPTS * CH; i += CONV_THREADS) {
-            const int tp = i / CH, c = i % CH;
-            const int iz = tp % TZT, iy = (tp / TZT) % TYT, ix = tp / (TZT * TYT);
-            const long long gp = (long long)b * N3 + ((long long)(x0 + ix) * N + (y0 + iy)) * N + (z0 + iz);
-            *(u32x4*)(a.act_out + gp * COUT + c * 8) = *(const u32x4*)(lds + tp * OS + c * 8);
-        }
+    __syncthreads();
+    constexpr int CH = COUT / 8;
+    for (int i = threadIdx.x; i < TPTS * CH; i += CONV_THREADS) {
+        const int tp = i / CH, c = i % CH;
+        const int iz = tp % TZT, iy = (tp / TZT) % TYT, ix = tp / (TZT * TYT);
+        const long long gp = (long long)b * N3 + ((long long)(x0 + ix) * N + (y0 + iy)) * N + (z0 + iz);
+        *(u32x4*)(a.act_out + gp * COUT + c * 8) = *(const u32x4*)(lds + tp * OS + c * 8);
     }
 }
 
