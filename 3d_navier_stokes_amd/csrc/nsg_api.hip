@@ -31,8 +31,10 @@ using namespace nsg;
 // Launch schedules = FFT schedule x persistent z-pass grid.
 // FFT schedules.  EX/EY/EZ: elements per thread of the x, y (length N) and z (length N/2)
 // transforms, which fixes the radix sequence (nsg_fft.hpp); WX/WY: kz columns per block of the x
-// and y passes.  FFT schedule 0 is the default; the ensemble sizes 32..96 have two alternatives:
-// 1 = shorter radices (more threads per transform), 2 = wider column tiles.
+// and y passes.  FFT schedule 0 is the default; the sizes 32..256 have two alternatives:
+// 1 = shorter radices (more threads per transform), 2 = wider column tiles.  512 and 1024 keep
+// their per-pass layouts (pass 5's LDS-pipelined reads need 128-byte rows, and the alternatives
+// measured there are the ones the comments below record).
 // z grids: the persistent z pass runs kZGrid[g] blocks per CU (capped by its work); the grid
 // changes no result bit (every line is transformed the same way whichever wavefront takes it, and
 // the per-box max is exact).  Schedule s = g * (FFT schedules of N) + f; nsg_autotune times all.
@@ -60,7 +62,7 @@ constexpr SchedV sched0(int N) {
            : N == 512 ? SchedV{8, 8, 8, 16, 4, 8, 8, 16, 8, 16}
                       : SchedV{8, 8, 16, 16, 8, 8, 4, 8, 8, 16};
 }
-constexpr int nsched(int N) { return (N == 32 || N == 48 || N == 64 || N == 96) ? 3 : 1; }  // FFT schedules
+constexpr int nsched(int N) { return (N >= 32 && N <= 256) ? 3 : 1; }  // FFT schedules
 constexpr int kZGrid[] = {8, 4, 12};
 constexpr int kNZGrid = (int)(sizeof(kZGrid) / sizeof(kZGrid[0]));
 constexpr SchedV sched_of(int N, int S) {

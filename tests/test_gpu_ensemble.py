@@ -17,7 +17,7 @@ from conftest import rel
 
 pytestmark = pytest.mark.gpu
 
-ALT_NS = [32, 48, 64, 96]
+ALT_NS = [32, 48, 64, 96, 128, 256]
 
 
 def k4(oracle, N, seed, k0=2.0):
@@ -38,7 +38,7 @@ def test_schedule_rhs_parity(nsg, oracle, N, sched):
         assert rel(s.rhs(uh), want) < 2e-6
 
 
-@pytest.mark.parametrize("N", [32, 48])
+@pytest.mark.parametrize("N", [32, 48, 128])
 @pytest.mark.parametrize("sched", [1, 2])
 def test_schedule_steps_parity(nsg, oracle, N, sched):
     uh = k4(oracle, N, 5 + N)
@@ -51,6 +51,31 @@ def test_schedule_steps_parity(nsg, oracle, N, sched):
     assert rel(got - uh, want - uh) < 1e-12
 
 
+@pytest.mark.parametrize("sched", [1, 2])
+def test_schedule_steps_256(nsg, oracle, sched):
+    """the 256^3 FFT schedules 1 (4 elements per thread) and 2 (wider column tiles) in the stage
+    passes: two forced RK4 steps in FP64 agree with schedule 0's (itself at 1e-12 of the oracle's
+    right-hand side at 256^3) to 1e-12 on the increment, and FP32-bulk under the same schedule
+    agrees with that FP64 state to 1e-8, the one-step FP32-bulk bar against the oracle"""
+    N = 256
+    uh = k4(oracle, N, 21)
+    kw = dict(nu=1e-3, dt=1e-3, force_eps=0.1)
+    out = {}
+    for s_, prec in ((0, "fp64"), (sched, "fp64"), (sched, "fp32bulk")):
+        with nsg.Solver(N, precision=prec, **kw) as s:
+            s.set_schedule(s_)
+            s.set_state(uh)
+            s.step(2 if prec == "fp64" else 1)
+            out[s_, prec] = s.get_state()
+    want = out[0, "fp64"]
+    assert rel(out[sched, "fp64"] - uh, want - uh) < 1e-12
+    with nsg.Solver(N, **kw) as s:
+        s.set_state(uh)
+        s.step(1)
+        one = s.get_state()
+    assert rel(out[sched, "fp32bulk"], one) < 1e-8
+
+
 def test_schedule_out_of_range(nsg):
     with nsg.Solver(16) as s:
         with pytest.raises(nsg.NSGError):
@@ -60,20 +85,21 @@ def test_schedule_out_of_range(nsg):
 
 @pytest.mark.parametrize("N,prec", [(256, "fp32bulk"), (128, "fp64")])
 def test_z_grid_schedules_bitwise(nsg, oracle, N, prec):
-    """schedules 1 and 2 of a size without FFT alternatives differ from schedule 0 only in the
+    """launch schedules F and 2F (F = FFT schedules of N) differ from schedule 0 only in the
     persistent z-pass grid (4 and 12 blocks per CU instead of 8), which must change no bit; the
     launches differ here (N = 256 FP32: 8192 blocks of work on grids of 2048, 1024 and 3072 blocks;
     N = 128 FP64: 2048 blocks of work, grids of 2048, 1024 and 2048)"""
-    assert nsg.schedule_count(N) == 3
+    nf = nsg.schedule_count(N) // 3
+    assert nf == 3
     uh = k4(oracle, N, 11)
-    got = []
-    for sched in range(3):
+    got = {}
+    for sched in (0, nf, 2 * nf):
         with nsg.Solver(N, nu=1e-3, dt=1e-3, cfl=0.5, precision=prec) as s:
             s.set_schedule(sched)
             s.set_state(uh)
             s.step(2)
-            got.append((s.get_state(), s.time()))
-    for sched in (1, 2):
+            got[sched] = (s.get_state(), s.time())
+    for sched in (nf, 2 * nf):
         assert np.array_equal(got[sched][0], got[0][0]), f"schedule {sched}"
         assert got[sched][1] == got[0][1]
 
