@@ -35,7 +35,7 @@ using namespace nsg;
 // 1 = shorter radices (more threads per transform), 2 = wider column tiles.  512 and 1024 keep
 // their per-pass layouts (pass 5's LDS-pipelined reads need 128-byte rows, and the alternatives
 // measured there are the ones the comments below record).
-// z grids: the persistent z pass runs kZGrid[g] blocks per CU (capped by its work); the grid
+// z grids: the persistent z pass runs zgrid_of(g) blocks per CU (capped by its work); the grid
 // changes no result bit (every line is transformed the same way whichever wavefront takes it, and
 // the per-box max is exact).  Schedule s = g * (FFT schedules of N) + f; nsg_autotune times all.
 // ------------------------------------------------------------------------------------------
@@ -65,6 +65,13 @@ constexpr SchedV sched0(int N) {
 constexpr int nsched(int N) { return (N >= 32 && N <= 256) ? 3 : 1; }  // FFT schedules
 constexpr int kZGrid[] = {8, 4, 12};
 constexpr int kNZGrid = (int)(sizeof(kZGrid) / sizeof(kZGrid[0]));
+// z grid g of a size and precision: 128^3 FP64 (two resident blocks per CU, 2048 blocks of work)
+// defaults to 4 blocks per CU, so that each block takes two line groups: 0.400 against 0.446 ms
+// per step with 8 (autotuner timings, one forced box); every other size keeps 8 (at 512^3 the z
+// pass ran within 0.5 % of 8 with 6, 12 or 16 blocks per CU, and 1-9 % slower with 2, 3 or 4)
+template <typename T, int N> static constexpr int zgrid_of(int g) {
+    return (N == 128 && std::is_same<T, double>::value && g < 2) ? kZGrid[1 - g] : kZGrid[g];
+}
 constexpr SchedV sched_of(int N, int S) {
     SchedV v = sched0(N);
     if (S == 1) {
@@ -328,7 +335,7 @@ static int launch_passz_rhs(nsg_ctx* c, int b0, int nb, double scale, int track_
         if constexpr (zfull_pair<T, N>() && zfull_pairs<N, EF>()) iters = groups / 2;
     }
     long long blocks = (iters + WPB - 1) / WPB;
-    const int nf = nsched(N), zg = kZGrid[(c->sched / nf) % kNZGrid];
+    const int nf = nsched(N), zg = zgrid_of<T, N>((c->sched / nf) % kNZGrid);
     const long long cap = (long long)c->ncu * (c->zblocks_env > 0 ? c->zblocks_env : zg);
     if (blocks > cap) blocks = cap;
     prof_begin(c);

@@ -86,9 +86,9 @@ def test_schedule_out_of_range(nsg):
 @pytest.mark.parametrize("N,prec", [(256, "fp32bulk"), (128, "fp64")])
 def test_z_grid_schedules_bitwise(nsg, oracle, N, prec):
     """launch schedules F and 2F (F = FFT schedules of N) differ from schedule 0 only in the
-    persistent z-pass grid (4 and 12 blocks per CU instead of 8), which must change no bit; the
-    launches differ here (N = 256 FP32: 8192 blocks of work on grids of 2048, 1024 and 3072 blocks;
-    N = 128 FP64: 2048 blocks of work, grids of 2048, 1024 and 2048)"""
+    persistent z-pass grid (N = 256 FP32: 4 and 12 blocks per CU instead of 8, 8192 blocks of work
+    on grids of 2048, 1024 and 3072 blocks; N = 128 FP64: 8 and 12 instead of 4, 2048 blocks of
+    work on grids of 1024, 2048 and 2048), which must change no bit"""
     nf = nsg.schedule_count(N) // 3
     assert nf == 3
     uh = k4(oracle, N, 11)
