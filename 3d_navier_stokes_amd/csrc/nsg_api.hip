@@ -31,8 +31,8 @@ using namespace nsg;
 // Launch schedules = FFT schedule x persistent z-pass grid.
 // FFT schedules.  EX/EY/EZ: elements per thread of the x, y (length N) and z (length N/2)
 // transforms, which fixes the radix sequence (nsg_fft.hpp); WX/WY: kz columns per block of the x
-// and y passes.  FFT schedule 0 is the default; the sizes 32..256 have two alternatives:
-// 1 = shorter radices (more threads per transform), 2 = wider column tiles.  512 and 1024 keep
+// and y passes.  FFT schedule 0 is the default; the sizes up to 256 have two alternatives:
+// 1 = shorter radices (more threads per transform; up to 24, longer ones), 2 = wider column tiles.  512 and 1024 keep
 // their per-pass layouts (pass 5's LDS-pipelined reads need 128-byte rows, and the alternatives
 // measured there are the ones the comments below record).
 // z grids: the persistent z pass runs zgrid_of(g) blocks per CU (capped by its work); the grid
@@ -43,16 +43,19 @@ struct SchedV {
     int ex64, ex32, ey64, ey32, ez64, ez32, wx64, wx32, wy64, wy32;
 };
 constexpr int nzp_of(int N) { return ((N / 3 + 1 + 15) / 16) * 16; }  // = Ext<N>::NZP
+// 8..24: 4 or 6 elements per thread in the x and y passes (2-4 threads per line); one-box steps
+// (autotuner timings, both precisions) ran 6 % (8^3), 16 % (12^3), 18 % (16^3) and 21 % (24^3)
+// faster than with one or two threads per line, which is now FFT schedule 1 there.
 // 1024: pass 1 takes 8 elements per thread in 4 (FP64) or 8 (FP32) columns, 64-byte row segments
 // whose two halves go to blocks launched 8 apart (xgrid); with 16 elements in 8 / 16 columns it
 // spilled 630-810 bytes per lane (measured 1024^3 step: FP64 4.18 -> 4.63, FP32-bulk 6.75 -> 8.84
 // steps/s with pass 5 as set by E5of / W5of)
 constexpr SchedV sched0(int N) {
     //                     ex64 ex32 ey64 ey32 ez64 ez32 wx64 wx32 wy64 wy32
-    return N == 8     ? SchedV{8, 8, 8, 8, 4, 4, 8, 16, 8, 16}
-           : N == 12  ? SchedV{12, 12, 12, 12, 6, 6, 8, 16, 8, 16}
-           : N == 16  ? SchedV{8, 8, 8, 8, 4, 8, 8, 16, 8, 16}
-           : N == 24  ? SchedV{12, 12, 12, 12, 12, 12, 8, 16, 8, 16}
+    return N == 8     ? SchedV{4, 4, 4, 4, 4, 4, 8, 16, 8, 16}
+           : N == 12  ? SchedV{6, 6, 6, 6, 6, 6, 8, 16, 8, 16}
+           : N == 16  ? SchedV{4, 4, 4, 4, 4, 8, 8, 16, 8, 16}
+           : N == 24  ? SchedV{6, 6, 6, 6, 12, 12, 8, 16, 8, 16}
            : N == 32  ? SchedV{8, 8, 8, 8, 8, 8, 8, 16, 8, 16}
            : N == 48  ? SchedV{12, 12, 12, 12, 12, 12, 8, 16, 8, 16}
            : N == 64  ? SchedV{8, 8, 8, 8, 8, 8, 8, 16, 8, 16}
@@ -62,7 +65,7 @@ constexpr SchedV sched0(int N) {
            : N == 512 ? SchedV{8, 8, 8, 16, 4, 8, 8, 16, 8, 16}
                       : SchedV{8, 8, 16, 16, 8, 8, 4, 8, 8, 16};
 }
-constexpr int nsched(int N) { return (N >= 32 && N <= 256) ? 3 : 1; }  // FFT schedules
+constexpr int nsched(int N) { return N <= 256 ? 3 : 1; }  // FFT schedules
 constexpr int kZGrid[] = {8, 4, 12};
 constexpr int kNZGrid = (int)(sizeof(kZGrid) / sizeof(kZGrid[0]));
 // z grid g of a size and precision: 128^3 FP64 (two resident blocks per CU, 2048 blocks of work)
@@ -74,7 +77,11 @@ template <typename T, int N> static constexpr int zgrid_of(int g) {
 }
 constexpr SchedV sched_of(int N, int S) {
     SchedV v = sched0(N);
-    if (S == 1) {
+    if (S == 1 && N <= 24) {
+        // the sizes whose default already takes the short radices: one (8, 12) or two (16, 24)
+        // threads per x and y line instead
+        v.ex64 = v.ex32 = v.ey64 = v.ey32 = N <= 12 ? N : N / 2;
+    } else if (S == 1) {
         const int e = (N % 3 == 0) ? 6 : 4;
         v.ex64 = v.ex32 = v.ey64 = v.ey32 = v.ez64 = v.ez32 = e;
     } else if (S == 2) {

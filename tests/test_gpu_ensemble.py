@@ -17,7 +17,7 @@ from conftest import rel
 
 pytestmark = pytest.mark.gpu
 
-ALT_NS = [32, 48, 64, 96, 128, 256]
+ALT_NS = [8, 12, 16, 24, 32, 48, 64, 96, 128, 256]
 
 
 def k4(oracle, N, seed, k0=2.0):
@@ -38,7 +38,7 @@ def test_schedule_rhs_parity(nsg, oracle, N, sched):
         assert rel(s.rhs(uh), want) < 2e-6
 
 
-@pytest.mark.parametrize("N", [32, 48, 128])
+@pytest.mark.parametrize("N", [12, 16, 32, 48, 128])
 @pytest.mark.parametrize("sched", [1, 2])
 def test_schedule_steps_parity(nsg, oracle, N, sched):
     uh = k4(oracle, N, 5 + N)
@@ -79,8 +79,10 @@ def test_schedule_steps_256(nsg, oracle, sched):
 def test_schedule_out_of_range(nsg):
     with nsg.Solver(16) as s:
         with pytest.raises(nsg.NSGError):
-            s.set_schedule(3)
-    assert nsg.schedule_count(16) == 3 and nsg.schedule_count(17) == 0
+            s.set_schedule(9)
+        with pytest.raises(nsg.NSGError):
+            s.set_schedule(-1)
+    assert nsg.schedule_count(16) == 9 and nsg.schedule_count(512) == 3 and nsg.schedule_count(17) == 0
 
 
 @pytest.mark.parametrize("N,prec", [(256, "fp32bulk"), (128, "fp64")])
