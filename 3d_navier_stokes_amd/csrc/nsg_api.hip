@@ -32,9 +32,9 @@ using namespace nsg;
 // FFT schedules.  EX/EY/EZ: elements per thread of the x, y (length N) and z (length N/2)
 // transforms, which fixes the radix sequence (nsg_fft.hpp); WX/WY: kz columns per block of the x
 // and y passes.  FFT schedule 0 is the default; the sizes up to 256 have two alternatives:
-// 1 = shorter radices (more threads per transform; up to 24, longer ones), 2 = wider column tiles.  512 and 1024 keep
-// their per-pass layouts (pass 5's LDS-pipelined reads need 128-byte rows, and the alternatives
-// measured there are the ones the comments below record).
+// 1 = the other radices (shorter, or longer where schedule 0 is short), 2 = wider column tiles.
+// 512 and 1024 keep their per-pass layouts (pass 5's LDS-pipelined reads need 128-byte rows, and
+// the alternatives measured there are the ones the comments below record).
 // z grids: the persistent z pass runs zgrid_of(g) blocks per CU (capped by its work); the grid
 // changes no result bit (every line is transformed the same way whichever wavefront takes it, and
 // the per-box max is exact).  Schedule s = g * (FFT schedules of N) + f; nsg_autotune times all.
@@ -43,19 +43,18 @@ struct SchedV {
     int ex64, ex32, ey64, ey32, ez64, ez32, wx64, wx32, wy64, wy32;
 };
 constexpr int nzp_of(int N) { return ((N / 3 + 1 + 15) / 16) * 16; }  // = Ext<N>::NZP
-// 8..24: 4 or 6 elements per thread in the x and y passes (2-4 threads per line); one-box steps
-// (autotuner timings, both precisions) ran 6 % (8^3), 16 % (12^3), 18 % (16^3) and 21 % (24^3)
-// faster than with one or two threads per line, which is now FFT schedule 1 there.
 // 1024: pass 1 takes 8 elements per thread in 4 (FP64) or 8 (FP32) columns, 64-byte row segments
 // whose two halves go to blocks launched 8 apart (xgrid); with 16 elements in 8 / 16 columns it
 // spilled 630-810 bytes per lane (measured 1024^3 step: FP64 4.18 -> 4.63, FP32-bulk 6.75 -> 8.84
-// steps/s with pass 5 as set by E5of / W5of)
-constexpr SchedV sched0(int N) {
+// steps/s with pass 5 as set by E5of / W5of).  Up to 96 this table holds the long radices (one
+// thread per line at 8 and 12, 8 or 12 elements per thread above); sched0 shortens them where
+// that measured faster.
+constexpr SchedV sched_long(int N) {
     //                     ex64 ex32 ey64 ey32 ez64 ez32 wx64 wx32 wy64 wy32
-    return N == 8     ? SchedV{4, 4, 4, 4, 4, 4, 8, 16, 8, 16}
-           : N == 12  ? SchedV{6, 6, 6, 6, 6, 6, 8, 16, 8, 16}
-           : N == 16  ? SchedV{4, 4, 4, 4, 4, 8, 8, 16, 8, 16}
-           : N == 24  ? SchedV{6, 6, 6, 6, 12, 12, 8, 16, 8, 16}
+    return N == 8     ? SchedV{8, 8, 8, 8, 4, 4, 8, 16, 8, 16}
+           : N == 12  ? SchedV{12, 12, 12, 12, 6, 6, 8, 16, 8, 16}
+           : N == 16  ? SchedV{8, 8, 8, 8, 4, 8, 8, 16, 8, 16}
+           : N == 24  ? SchedV{12, 12, 12, 12, 12, 12, 8, 16, 8, 16}
            : N == 32  ? SchedV{8, 8, 8, 8, 8, 8, 8, 16, 8, 16}
            : N == 48  ? SchedV{12, 12, 12, 12, 12, 12, 8, 16, 8, 16}
            : N == 64  ? SchedV{8, 8, 8, 8, 8, 8, 8, 16, 8, 16}
@@ -64,6 +63,21 @@ constexpr SchedV sched0(int N) {
            : N == 256 ? SchedV{8, 8, 8, 8, 4, 8, 8, 16, 8, 16}
            : N == 512 ? SchedV{8, 8, 8, 16, 4, 8, 8, 16, 8, 16}
                       : SchedV{8, 8, 16, 16, 8, 8, 4, 8, 8, 16};
+}
+constexpr int short_ept(int N) { return (N % 3 == 0) ? 6 : 4; }
+// Where schedule 0 takes the short radices (4 or 6 elements per thread in the x and y passes).
+// One-box steps (autotuner timings) with them against the long ones: FP64 and FP32 8^3 0.083 vs
+// 0.088 ms, 12^3 0.118 vs 0.140, 16^3 0.089 vs 0.108, 24^3 0.135 vs 0.170; FP64 32^3 0.107 vs
+// 0.115, 48^3 0.169 vs 0.200, 64^3 0.165 vs 0.171, 96^3 0.345 vs 0.370; FP32 48^3 0.154 vs 0.160,
+// 64^3 0.132 vs 0.136, 96^3 0.270 vs 0.277, but 32^3 0.115 vs 0.108 (long kept); 128^3 and 256^3
+// were faster long in both precisions.
+constexpr bool short64(int N) { return N <= 96; }
+constexpr bool short32(int N) { return N <= 24 || (N >= 48 && N <= 96); }
+constexpr SchedV sched0(int N) {
+    SchedV v = sched_long(N);
+    if (short64(N)) v.ex64 = v.ey64 = short_ept(N);
+    if (short32(N)) v.ex32 = v.ey32 = short_ept(N);
+    return v;
 }
 constexpr int nsched(int N) { return N <= 256 ? 3 : 1; }  // FFT schedules
 constexpr int kZGrid[] = {8, 4, 12};
@@ -76,15 +90,16 @@ template <typename T, int N> static constexpr int zgrid_of(int g) {
     return (N == 128 && std::is_same<T, double>::value && g < 2) ? kZGrid[1 - g] : kZGrid[g];
 }
 constexpr SchedV sched_of(int N, int S) {
+    if (S == 1) {
+        // the other radices: the long ones where schedule 0 is short, the short ones elsewhere
+        SchedV v = sched_long(N);
+        const int e = short_ept(N);
+        if (!short64(N)) v.ex64 = v.ey64 = v.ez64 = e;
+        if (!short32(N)) v.ex32 = v.ey32 = v.ez32 = e;
+        return v;
+    }
     SchedV v = sched0(N);
-    if (S == 1 && N <= 24) {
-        // the sizes whose default already takes the short radices: one (8, 12) or two (16, 24)
-        // threads per x and y line instead
-        v.ex64 = v.ex32 = v.ey64 = v.ey32 = N <= 12 ? N : N / 2;
-    } else if (S == 1) {
-        const int e = (N % 3 == 0) ? 6 : 4;
-        v.ex64 = v.ex32 = v.ey64 = v.ey32 = v.ez64 = v.ez32 = e;
-    } else if (S == 2) {
+    if (S == 2) {
         v.wx64 = v.wy64 = 16;
         v.wx32 = v.wy32 = (nzp_of(N) % 32 == 0) ? 32 : 16;
     }
