@@ -133,11 +133,11 @@ int nsg_profile_passes(nsg_ctx* ctx, int iters, double* ms, double* bytes);
 double nsg_step_bytes(const nsg_ctx* ctx);
 
 /* Launch schedules = FFT schedule f x persistent z-pass grid g, numbered s = g * F + f with F the
- * FFT schedules of N.  Every size has FFT schedule 0; N <= 256 have two alternatives (1:
- * shorter radices, more threads per transform, or at N <= 24 longer ones; 2: wider column tiles), so
- * F = 3 there and 1 elsewhere.
- * The z grids are 8, 4 and 12 blocks per CU (g = 0, 1, 2; 4, 8, 12 at 128^3 FP64).  FFT schedules
- * differ only in rounding; the z grid changes no result bit.  nsg_schedule_count(N) = 3 F.  nsg_autotune times `iters` steps
+ * FFT schedules of N.  Every size has FFT schedule 0; N <= 256 have two alternatives (1: the other
+ * x/y radices, shorter or longer than schedule 0's; 2: wider column tiles), so F = 3 there and 1
+ * elsewhere.  The z grids are 8, 4 and 12 blocks per CU (g = 0, 1, 2; 4, 8, 12 at 128^3 FP64).
+ * FFT schedules differ only in rounding; the z grid changes no result bit.
+ * nsg_schedule_count(N) = 3 F.  nsg_autotune times `iters` steps
  * of each on the context's own state, restores the state and time afterwards, keeps the fastest and
  * caches the choice per (N, precision, boxes) for the process (use_cache = 0 re-measures).
  * ms_per_step, if not NULL, receives one entry per schedule.

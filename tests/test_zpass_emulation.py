@@ -54,8 +54,8 @@ def reference(a2, N, kmax, nzp):
     return out, mx
 
 
-# (N, elements per thread) of the library's schedules for N <= 128 (schedule 0 and the shorter-radix
-# schedule 1), per precision; (16, 8) in FP64 is the removed instance
+# (N, elements per thread) of the half-length z transforms of the library's schedules for N <= 128
+# (the two radix choices of FFT schedules 0 and 1), per precision; (16, 8) in FP64 is the removed instance
 CASES = [("fp64", 8, 4), ("fp64", 16, 4), ("fp64", 16, 8), ("fp64", 32, 8), ("fp64", 32, 4), ("fp64", 64, 8),
          ("fp64", 64, 4), ("fp64", 128, 8), ("fp32", 8, 4), ("fp32", 16, 8), ("fp32", 32, 8), ("fp32", 32, 4),
          ("fp32", 64, 8), ("fp32", 64, 4), ("fp32", 128, 8), ("fp32", 128, 16), ("fp64", 128, 16)]
