@@ -46,9 +46,10 @@ constexpr int nzp_of(int N) { return ((N / 3 + 1 + 15) / 16) * 16; }  // = Ext<N
 // 1024: pass 1 takes 8 elements per thread in 4 (FP64) or 8 (FP32) columns, 64-byte row segments
 // whose two halves go to blocks launched 8 apart (xgrid); with 16 elements in 8 / 16 columns it
 // spilled 630-810 bytes per lane (measured 1024^3 step: FP64 4.18 -> 4.63, FP32-bulk 6.75 -> 8.84
-// steps/s with pass 5 as set by E5of / W5of).  Up to 96 this table holds the long radices (one
-// thread per line at 8 and 12, 8 or 12 elements per thread above); sched0 shortens them where
-// that measured faster.
+// steps/s with pass 5 as set by E5of / W5of).  512 FP32: pass 1 with 16 elements per thread in
+// 512-thread blocks (spill-free; the 1024-thread form spills 12 bytes) ran 0.715 against 0.644 ms.
+// Up to 96 this table holds the long radices (one thread per line at 8 and 12, 8 or 12 elements
+// per thread above); sched0 shortens them where that measured faster.
 constexpr SchedV sched_long(int N) {
     //                     ex64 ex32 ey64 ey32 ez64 ez32 wx64 wx32 wy64 wy32
     return N == 8     ? SchedV{8, 8, 8, 8, 4, 4, 8, 16, 8, 16}
